@@ -1,0 +1,183 @@
+/* shrt.h -- C ABI of the MI355X-native shearing-rt pseudospectral step.
+ *
+ * The reference (Fulac/shearing-rt) has no plugin or FFI layer: its hot path is a
+ * set of C++ functions on global state (time_integral.h, fft.h, shear.h,
+ * fields.h) driven by main.cu.  Each entry point below replaces one of them; the
+ * replaced reference interface is cited on every declaration.  Plain C types
+ * only, no torch or HIP types.  The library owns device memory (HBM); the caller
+ * owns host buffers, which use the reference's layouts:
+ *
+ *   spectral  f[ikx*nky + iky], complex interleaved {re, im}   (fields.cu:137-165)
+ *             kx order 0..nkxh-1, -(nkxh-1)..-1                 (fourier.cu:124-125)
+ *   physical  f[ix*ny + iy], real                              (fft.cu:160-164)
+ *
+ * The element type of host buffers is the context's precision ("cureal",
+ * cmplx.h:7-13): float / float2 for SHRT_FP32, double / double2 for SHRT_FP64.
+ * Precision is chosen per context at create time (it replaces -DDBLE).
+ * Shift bookkeeping (ky_shift, remap trigger and offsets) and the time are FP64
+ * in both precisions.
+ *
+ * Every call returns SHRT_OK (0) or an error code; nothing calls exit().  The
+ * text of the last error of the calling thread is in shrt_last_error().  A
+ * context is used from one host thread; its GPU work is ordered on its own
+ * HIP stream.
+ */
+#ifndef SHRT_H
+#define SHRT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    SHRT_OK = 0,
+    SHRT_ERR_INVALID = 1,      /* bad argument or unsupported configuration        */
+    SHRT_ERR_UNSUPPORTED = 2,  /* grid size outside the compiled transform lengths */
+    SHRT_ERR_DEVICE = 3,       /* a HIP call failed                                */
+    SHRT_ERR_ALLOC = 4,        /* device memory allocation failed                  */
+    SHRT_ERR_STATE = 5         /* the device reported an error during a step       */
+} shrt_status;
+
+typedef enum { SHRT_FP32 = 4, SHRT_FP64 = 8 } shrt_precision;
+
+/* Spectral fields (the reference's dv_aomg0 / dv_aphi / dv_arho0). */
+typedef enum { SHRT_OMEGA = 0, SHRT_PHI = 1, SHRT_RHO = 2 } shrt_field;
+
+/* Physical fields: omega, phi, rho (output_fields, file_access.cu:250-263) and the
+ * velocities vx, vy of the last CFL check (time_integral.cu:354). */
+typedef enum { SHRT_P_OMEGA = 0, SHRT_P_PHI = 1, SHRT_P_RHO = 2, SHRT_P_VX = 3, SHRT_P_VY = 4 } shrt_pfield;
+
+/* Ring buffers of the AB3/BDF2 history (fields.cu:21-25). */
+typedef enum { SHRT_RING_AOMG = 0, SHRT_RING_DOMG = 1, SHRT_RING_ARHO = 2, SHRT_RING_DRHO = 3 } shrt_ring;
+
+/* Parameters of input_data (file_access.cu:123-165), with the same meaning.
+ * `delt` is the initial time step; check_cfl may halve it during a step. */
+typedef struct {
+    int nx, ny;
+    double Lx, Ly;
+    double delt;
+    double cfl_num;
+    double nu, kappa, sigma, g, rho0, rho0_prime, rho_eps1;
+    int noise;        /* "initial noise" */
+    int linear;       /* "linear eq"     */
+    int precision;    /* SHRT_FP32 or SHRT_FP64 */
+    int graph_steps;  /* steps per captured HIP graph (even, >= 2); 0 = launch eagerly */
+} shrt_params;
+
+/* Seeded initial condition (new; additive to the reference IC):
+ * omega = 0; on retained modes with kmin <= |k| <= kmax,
+ * rho_hat = amp * |k|^expo * exp(i*theta), theta from SplitMix64(seed). */
+typedef struct {
+    uint64_t seed;
+    double kmin, kmax, amp, expo;
+} shrt_seeded_ic;
+
+typedef struct {
+    int64_t istep;          /* number of completed steps (main.cu:14)          */
+    double time;            /* simulated time (time_integral.cu:340)           */
+    double delt;            /* current time step after CFL halvings            */
+    int remap_count;        /* number of remap events (shear.cu:234)           */
+    int halvings;           /* number of CFL halvings (time_integral.cu:361)   */
+    int64_t last_remap_step;/* step index of the last remap, -1 if none        */
+} shrt_time;
+
+typedef struct {
+    double energy;          /* 1/2 sum w kperp2_shear |phi_hat|^2  (FP64, compensated) */
+    double enstrophy;       /* 1/2 sum w |omega_hat|^2                                   */
+    double rho_var;         /* 1/2 sum w |rho_hat|^2                                     */
+} shrt_diag;
+
+typedef struct {
+    int64_t istep;
+    double time, value, delt;
+    int which;              /* 0 cfl_vx, 1 cfl_vy, 2 g/rho0, 3 rho0_prime */
+    int count;              /* halvings taken in that step                */
+} shrt_halving;
+
+typedef struct shrt_ctx shrt_ctx;
+
+/* init_four + init_fields + init_tint + init_shear (main.cu:28-32; fourier.cu:116-150,
+ * fields.cu:68-81, time_integral.cu:205-215, shear.cu:149-185).  All buffers zeroed. */
+shrt_status shrt_create(const shrt_params* params, shrt_ctx** out);
+
+/* finish_* (main.cu:69-73). */
+shrt_status shrt_destroy(shrt_ctx* ctx);
+
+/* initialize() (time_integral.cu:229-259): rho = rho_eps1*cos(2x+10y), omega = 0,
+ * xtok, optional glibc-rand() noise (init_dis, :261-320), aphi = omega/kperp2. */
+shrt_status shrt_initialize(shrt_ctx* ctx);
+
+/* Seeded initial condition (new). */
+shrt_status shrt_initialize_seeded(shrt_ctx* ctx, const shrt_seeded_ic* ic);
+
+/* time_advance(istep, time) (time_integral.cu:324-341), `nsteps` times.  Returns
+ * after the device has finished and reports device-side errors. */
+shrt_status shrt_step(shrt_ctx* ctx, int nsteps);
+
+/* Enqueue `nsteps` steps on the context's stream and return at once (no sync,
+ * no error check); pair with shrt_synchronize(). */
+shrt_status shrt_step_async(shrt_ctx* ctx, int nsteps);
+shrt_status shrt_synchronize(shrt_ctx* ctx);
+
+/* Read / write the current spectral field (dv_aomg0, dv_aphi, dv_arho0), host
+ * buffer of nkx*nky complex.  Setting omega also sets aphi = omega/kperp2_shear
+ * (as initialize() does, time_integral.cu:257). */
+shrt_status shrt_get_spectral(shrt_ctx* ctx, int field, void* host);
+shrt_status shrt_set_spectral(shrt_ctx* ctx, int field, const void* host);
+
+/* Ring access by the reference's names after the last rotation: slot 0 = *0,
+ * 1 = *1, 2 = *2 (renew_fields, time_integral.cu:474-484). */
+shrt_status shrt_get_buffer(shrt_ctx* ctx, int ring, int slot, void* host);
+shrt_status shrt_set_buffer(shrt_ctx* ctx, int ring, int slot, const void* host);
+
+/* Physical field (ktox, fft.cu:170-204) of omega, phi, rho or the velocities, host
+ * buffer of nx*ny reals.  lab_frame must be 0 (the lab-frame transform
+ * ktox_shear, shear.cu:467-478, is not implemented yet: SHRT_ERR_UNSUPPORTED). */
+shrt_status shrt_get_physical(shrt_ctx* ctx, int pfield, int lab_frame, void* host);
+
+/* xtok / ktox (fft.h:17-25) on host buffers. */
+shrt_status shrt_xtok(shrt_ctx* ctx, const void* phys_in, void* spec_out);
+shrt_status shrt_ktox(shrt_ctx* ctx, const void* spec_in, void* phys_out);
+
+/* shearing_field (shear.cu:296-322): remap one spectral buffer with the given
+ * jump[nkx] (ikx_indexed fold for jump > 0). */
+shrt_status shrt_shearing_field(shrt_ctx* ctx, const int* jump, const void* in, void* out);
+
+/* Time and shift bookkeeping: time / istep / delt, and ky_shift[nkx] (FP64) plus the
+ * jump[nkx] of the last remap (shear.cu:21-23). */
+shrt_status shrt_get_time(shrt_ctx* ctx, shrt_time* out);
+shrt_status shrt_set_time(shrt_ctx* ctx, int64_t istep, double time, double delt);
+shrt_status shrt_get_shift(shrt_ctx* ctx, double* ky_shift, int* jump);
+shrt_status shrt_set_shift(shrt_ctx* ctx, const double* ky_shift);
+
+/* FP64 compensated energy / enstrophy / density variance (new). */
+shrt_status shrt_diagnostics(shrt_ctx* ctx, shrt_diag* out);
+
+/* CFL halving events recorded on the device (the printf lines of check_cfl,
+ * time_integral.cu:363-382).  Returns the number copied (<= cap) in *count. */
+shrt_status shrt_get_halvings(shrt_ctx* ctx, shrt_halving* out, int cap, int* count);
+
+/* Sizes derived by init_fft (fft.cu:91-96): out[0..7] = nkx, nky, nkxh, nkxh2,
+ * nkxpad, ncy, nkx*nky, nkxh*ny. */
+shrt_status shrt_sizes(int nx, int ny, int* out8);
+
+/* Whether the library can run a grid: 1 if nx, ny are supported powers of two. */
+int shrt_supported(int nx, int ny, int precision);
+
+/* Number of visible HIP devices (0 when none or the runtime is unavailable).
+ * Library utility with no reference counterpart. */
+int shrt_device_count(void);
+
+/* Last error message of the calling thread ("" if none). */
+const char* shrt_last_error(void);
+
+/* Library version string. */
+const char* shrt_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SHRT_H */

@@ -1,0 +1,302 @@
+// aux_kernels.hpp -- transforms and reductions outside the step:
+//   xtok (fft.cu:136-166)   = k_aux_xfwd then k_aux_yfwd
+//   ktox (fft.cu:170-204)   = k_aux_yinv then k_aux_xinv, with optional ik
+//                             multipliers for the velocities (shear.cu:378-391)
+//   shearing_field view     (shear.cu:296-322)
+//   neg_lapinv_shear        (shear.cu:367-374) for aphi after initialisation
+//   FP64 compensated energy / enstrophy / density-variance reductions (new)
+// They reuse the team FFT and the G/H tile layout of the step kernels.
+#pragma once
+
+#include "step_common.hpp"
+
+namespace shrt {
+
+template <typename T>
+struct AuxArgs {
+    Geo g;
+    const void* in;
+    void* out;
+    const double* kx;
+    const double* ky;
+    const double* s;      // shift table for the multipliers
+    const int* jmp;       // remap view of `in` (spectral inputs), or null
+    const cplx<T>* tw;
+    int mode;             // k_aux_yinv: 0 identity, 1 i*kys*F (vx of phi), 2 -i*kx*F (vy of phi)
+};
+
+// x-forward of one real field f[ix*ny + iy] for a tile of 4 y columns -> H tile.
+template <typename T, int NX>
+__global__ void __launch_bounds__(kTile * TeamShape<NX>::TT)
+k_aux_xfwd(AuxArgs<T> a) {
+    using S = TeamShape<NX>;
+    constexpr int P = S::P, TT = S::TT, L = lds_len<NX>();
+    constexpr int NQ = (NX / 3 + 1 + TT - 1) / TT;
+    __shared__ cplx<T> sm[kTile * L];
+    const Geo& g = a.g;
+    const int team = threadIdx.x / TT, tid = threadIdx.x - team * TT;
+    const int tile = blockIdx.x, y = tile * kTile + team, nk = g.nkxh;
+    const T* f = static_cast<const T*>(a.in);
+    cplx<T>* lds = sm + team * L;
+    auto sync = [] { __syncthreads(); };
+    cplx<T> v[P];
+#pragma unroll
+    for (int m = 0; m < P; ++m) v[m] = {f[(long long)(tid + m * TT) * g.ny + y], T(0)};
+    TeamFFT<T, NX, P, S::RMAX, -1>::run(v, lds, a.tw, tid, sync);
+#pragma unroll
+    for (int m = 0; m < P; ++m) lds[tid + m * TT] = v[m];
+    __syncthreads();
+    cplx<T> h[NQ];
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+        const int kx = tid + i * TT;
+        if (kx < nk) h[i] = lds[kx];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+        const int kx = tid + i * TT;
+        if (kx < nk) sm[kx * 4 + team] = h[i];
+    }
+    __syncthreads();
+    cplx<T>* Ho = static_cast<cplx<T>*>(a.out) + (long long)tile * nk * 4;
+    for (int idx = threadIdx.x; idx < nk * 4; idx += blockDim.x) Ho[idx] = sm[idx];
+}
+
+// y-forward of an H field for a tile of kx>=0 rows -> F (reference layout),
+// truncated and scaled (scale_dealias).
+template <typename T, int NY>
+__global__ void __launch_bounds__(kTile * TeamShape<NY>::TT)
+k_aux_yfwd(AuxArgs<T> a) {
+    using S = TeamShape<NY>;
+    constexpr int P = S::P, TT = S::TT, L = lds_len<NY>();
+    __shared__ cplx<T> sm[kTile * L];
+    const Geo& g = a.g;
+    const int team = threadIdx.x / TT, tid = threadIdx.x - team * TT;
+    const int tile = blockIdx.x, row = tile * kTile + team;
+    const bool act = row < g.nkxh;
+    const int rn = row == 0 ? 0 : g.nkx - row;
+    const int nrow_tile = min(kTile, g.nkxh - tile * kTile);
+    const cplx<T>* Hf = static_cast<const cplx<T>*>(a.in);
+    cplx<T>* F = static_cast<cplx<T>*>(a.out);
+    cplx<T>* lds = sm + team * L;
+    auto sync = [] { __syncthreads(); };
+    for (int idx = threadIdx.x; idx < NY * 4; idx += blockDim.x) {
+        const int t = idx >> 4, w = idx & 15;
+        sm[idx] = (w >> 2) < nrow_tile ? Hf[((long long)t * g.nkxh + tile * kTile) * 4 + w] : czero<T>();
+    }
+    __syncthreads();
+    cplx<T> v[P];
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+        const int e = tid + m * TT;
+        v[m] = sm[(e >> 2) * 16 + team * 4 + (e & 3)];
+    }
+    __syncthreads();
+    TeamFFT<T, NY, P, S::RMAX, -1>::run(v, lds, a.tw, tid, sync);
+#pragma unroll
+    for (int m = 0; m < P; ++m) lds[tid + m * TT] = v[m];
+    __syncthreads();
+    if (!act) return;
+    const T nn = (T)((double)g.nx * (double)g.ny);   // scale_dealias divides by nx*ny
+    for (int iky = tid; iky < g.nky; iky += TT) {
+        const cplx<T> p = lds[iky];
+        F[(long long)row * g.nky + iky] = {p.x / nn, p.y / nn};
+        if (row > 0) {
+            const cplx<T> q = lds[(NY - iky) & (NY - 1)];
+            F[(long long)rn * g.nky + iky] = {q.x / nn, -q.y / nn};
+        }
+    }
+}
+
+// y-inverse of a spectral field (reference layout, through the remap view),
+// times the multiplier of `mode`, for a tile of kx>=0 rows -> G tile.
+template <typename T, int NY>
+__global__ void __launch_bounds__(kTile * TeamShape<NY>::TT)
+k_aux_yinv(AuxArgs<T> a) {
+    using S = TeamShape<NY>;
+    constexpr int P = S::P, TT = S::TT, L = lds_len<NY>();
+    __shared__ cplx<T> sm[kTile * L];
+    const Geo& g = a.g;
+    const int team = threadIdx.x / TT, tid = threadIdx.x - team * TT;
+    const int tile = blockIdx.x, row = tile * kTile + team;
+    const bool act = row < g.nkxh;
+    const int rn = row == 0 ? 0 : g.nkx - row;
+    const int nrow_tile = min(kTile, g.nkxh - tile * kTile);
+    const cplx<T>* F = static_cast<const cplx<T>*>(a.in);
+    cplx<T>* lds = sm + team * L;
+    auto sync = [] { __syncthreads(); };
+    auto mult = [&](int r, int iky, cplx<T> z) -> cplx<T> {
+        if (a.mode == 1) {
+            const T k = (T)kys_of(a.ky, a.s, r, iky);
+            return {-k * z.y, k * z.x};
+        }
+        if (a.mode == 2) {
+            const T k = (T)a.kx[r];
+            return {k * z.y, -k * z.x};
+        }
+        return z;
+    };
+    cplx<T> v[P];
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+        const int e = tid + m * TT;
+        cplx<T> val = czero<T>();
+        if (act) {
+            if (e < g.nky) {
+                val = mult(row, e, view_load(F, row, e, g, a.jmp));
+                if (e == 0) {
+                    const cplx<T> vn = mult(rn, 0, view_load(F, rn, 0, g, a.jmp));
+                    val = {(val.x + vn.x) * T(0.5), (val.y - vn.y) * T(0.5)};
+                }
+            } else if (e > NY - g.nky) {
+                const int iky = NY - e;
+                const cplx<T> vn = mult(rn, iky, view_load(F, rn, iky, g, a.jmp));
+                val = {vn.x, -vn.y};
+            }
+        }
+        v[m] = val;
+    }
+    TeamFFT<T, NY, P, S::RMAX, +1>::run(v, lds, a.tw, tid, sync);
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+        const int e = tid + m * TT;
+        sm[(e >> 2) * 16 + team * 4 + (e & 3)] = v[m];
+    }
+    __syncthreads();
+    cplx<T>* Gf = static_cast<cplx<T>*>(a.out);
+    for (int idx = threadIdx.x; idx < NY * 4; idx += blockDim.x) {
+        const int t = idx >> 4, w = idx & 15;
+        if ((w >> 2) < nrow_tile) Gf[((long long)t * g.nkxh + tile * kTile) * 4 + w] = sm[idx];
+    }
+}
+
+// x-inverse (C2R) of a G field for a tile of 4 y columns -> real f[ix*ny + iy].
+template <typename T, int NX>
+__global__ void __launch_bounds__(kTile * TeamShape<NX>::TT)
+k_aux_xinv(AuxArgs<T> a) {
+    using S = TeamShape<NX>;
+    constexpr int P = S::P, TT = S::TT, L = lds_len<NX>();
+    __shared__ cplx<T> sm[kTile * L];
+    const Geo& g = a.g;
+    const int team = threadIdx.x / TT, tid = threadIdx.x - team * TT;
+    const int tile = blockIdx.x, y = tile * kTile + team, nk = g.nkxh;
+    const cplx<T>* Gf = static_cast<const cplx<T>*>(a.in) + (long long)tile * nk * 4;
+    T* f = static_cast<T*>(a.out);
+    cplx<T>* lds = sm + team * L;
+    auto sync = [] { __syncthreads(); };
+    for (int idx = threadIdx.x; idx < nk * 4; idx += blockDim.x) sm[idx] = Gf[idx];
+    __syncthreads();
+    cplx<T> v[P];
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+        const int e = tid + m * TT;
+        cplx<T> z = czero<T>();
+        if (e < nk) {
+            const cplx<T> A = sm[e * 4 + team];
+            z = e == 0 ? cplx<T>{A.x, T(0)} : A;
+        } else if (e > NX - nk) {
+            z = cconj(sm[(NX - e) * 4 + team]);
+        }
+        v[m] = z;
+    }
+    __syncthreads();
+    TeamFFT<T, NX, P, S::RMAX, +1>::run(v, lds, a.tw, tid, sync);
+#pragma unroll
+    for (int m = 0; m < P; ++m) f[(long long)(tid + m * TT) * g.ny + y] = v[m].x;
+}
+
+// out[t] = remap view of in (shear.cu:296-322); jmp null -> plain copy.
+template <typename T>
+__global__ void k_view_copy(Geo g, const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out,
+                            const int* __restrict__ jmp) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)g.nkx * g.nky) return;
+    const int row = (int)(t / g.nky), iky = (int)(t - (long long)row * g.nky);
+    out[t] = view_load(in, row, iky, g, jmp);
+}
+
+// aphi = omega / kperp2_shear for modes 1..M-1, aphi[0] = 0 (fourier.cu:233-240)
+template <typename T>
+__global__ void k_phi_from_omega(Geo g, const cplx<T>* __restrict__ om, cplx<T>* __restrict__ phi,
+                                 const double* kx, const double* ky, const double* s) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)g.nkx * g.nky) return;
+    const int row = (int)(t / g.nky), iky = (int)(t - (long long)row * g.nky);
+    if (t == 0) { phi[0] = czero<T>(); return; }
+    const T k2 = (T)k2_of(kx[row], kys_of(ky, s, row, iky));
+    const cplx<T> w = om[t];
+    phi[t] = {w.x / k2, w.y / k2};
+}
+
+template <typename T>
+__global__ void k_fill_zero(cplx<T>* p, long long n) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = czero<T>();
+}
+
+// Neumaier-compensated FP64 accumulation.
+struct KSum {
+    double s, c;
+    __device__ void add(double x) {
+        const double t = s + x;
+        c += fabs(s) >= fabs(x) ? (s - t) + x : (x - t) + s;
+        s = t;
+    }
+    __device__ void merge(KSum o) { add(o.s); add(o.c); }
+};
+
+constexpr int kDiagThreads = 256;
+constexpr int kDiagQ = 3;   // energy, enstrophy, density variance
+
+// Per-block partial sums of E = 1/2 sum w kperp2 |phi|^2, Z = 1/2 sum w |omega|^2,
+// R = 1/2 sum w |rho|^2 over the stored half plane (w = 1 for iky = 0, else 2).
+template <typename T>
+__global__ void __launch_bounds__(kDiagThreads)
+k_diag_partial(Geo g, const cplx<T>* __restrict__ om, const cplx<T>* __restrict__ rh, const int* jmp,
+               const double* kx, const double* ky, const double* s, double* partial) {
+    __shared__ double sh[kDiagQ * 2][kDiagThreads];
+    KSum acc[kDiagQ];
+#pragma unroll
+    for (int q = 0; q < kDiagQ; ++q) acc[q] = {0.0, 0.0};
+    const long long M = (long long)g.nkx * g.nky;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < M;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int row = (int)(t / g.nky), iky = (int)(t - (long long)row * g.nky);
+        const double w = iky == 0 ? 1.0 : 2.0;
+        const cplx<T> o = view_load(om, row, iky, g, jmp);
+        const cplx<T> r = view_load(rh, row, iky, g, jmp);
+        const double ox = o.x, oy = o.y, rx = r.x, ry = r.y;
+        const double o2 = ox * ox + oy * oy;
+        if (t > 0) {
+            const double k2 = k2_of(kx[row], kys_of(ky, s, row, iky));
+            acc[0].add(w * o2 / k2);   // kperp2 |omega/kperp2|^2
+        }
+        acc[1].add(w * o2);
+        acc[2].add(w * (rx * rx + ry * ry));
+    }
+#pragma unroll
+    for (int q = 0; q < kDiagQ; ++q) {
+        sh[2 * q][threadIdx.x] = acc[q].s;
+        sh[2 * q + 1][threadIdx.x] = acc[q].c;
+    }
+    __syncthreads();
+    if (threadIdx.x < kDiagQ) {
+        const int q = threadIdx.x;
+        KSum tot = {0.0, 0.0};
+        for (int i = 0; i < kDiagThreads; ++i) tot.merge({sh[2 * q][i], sh[2 * q + 1][i]});
+        partial[(long long)blockIdx.x * kDiagQ * 2 + 2 * q] = tot.s;
+        partial[(long long)blockIdx.x * kDiagQ * 2 + 2 * q + 1] = tot.c;
+    }
+}
+
+template <typename T>
+__global__ void k_diag_final(const double* partial, int nblocks, double* out) {
+    if (threadIdx.x >= kDiagQ) return;
+    const int q = threadIdx.x;
+    KSum tot = {0.0, 0.0};
+    for (int b = 0; b < nblocks; ++b) tot.merge({partial[b * kDiagQ * 2 + 2 * q], partial[b * kDiagQ * 2 + 2 * q + 1]});
+    out[q] = 0.5 * (tot.s + tot.c);
+}
+
+}  // namespace shrt

@@ -1,0 +1,203 @@
+// fft.hpp -- LDS-resident Stockham FFT for CDNA4 (gfx950), used by every
+// transform of the pseudospectral step.
+//
+// A "team" of TT = N/P threads transforms one length-N complex sequence.  Thread
+// t holds the P elements t + m*TT (m = 0..P-1) in registers, in natural order,
+// both on entry and on exit.  Each stage runs P/R radix-R butterflies per thread
+// in registers (R <= RMAX), writes its outputs to the team's LDS buffer at the
+// auto-sorted Stockham positions and reads its next inputs back in natural
+// order; the last stage needs no exchange.  Inter-stage twiddles come from a
+// per-length table W_N^k (rounded once from long double on the host), so the
+// transform is as accurate as the table and the radix kernels.
+//
+// Replaces the cuFFT 2-D R2C/C2R plans of fft.cu:111-119, 147-149, 183-185.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <type_traits>
+
+namespace shrt {
+
+template <typename T>
+struct alignas(2 * sizeof(T)) cplx {
+    T x, y;
+};
+
+template <typename T> __device__ __forceinline__ cplx<T> cadd(cplx<T> a, cplx<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T> __device__ __forceinline__ cplx<T> csub(cplx<T> a, cplx<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename T> __device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b) {
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename T> __device__ __forceinline__ cplx<T> cconj(cplx<T> a) { return {a.x, -a.y}; }
+template <typename T> __device__ __forceinline__ cplx<T> cscale(T s, cplx<T> a) { return {s * a.x, s * a.y}; }
+// a * (i*d), d = +1 or -1
+template <int D, typename T> __device__ __forceinline__ cplx<T> cmul_i(cplx<T> a) {
+    if constexpr (D > 0) return {-a.y, a.x};
+    else return {a.y, -a.x};
+}
+
+// Compile-time loop: f(std::integral_constant<int, I>) for I in [B, E).
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
+// cos(2*pi*j/32) for j = 0..8; other angles follow by symmetry.
+__host__ __device__ constexpr double cos32_q(int j) {
+    return j == 0 ? 1.0
+         : j == 1 ? 0.98078528040323044912618
+         : j == 2 ? 0.92387953251128675612818
+         : j == 3 ? 0.83146961230254523707879
+         : j == 4 ? 0.70710678118654752440084
+         : j == 5 ? 0.55557023301960222474283
+         : j == 6 ? 0.38268343236508977172846
+         : j == 7 ? 0.19509032201612826784828
+         : 0.0;
+}
+__host__ __device__ constexpr double cos32(int j) {
+    j = ((j % 32) + 32) % 32;
+    return j <= 8 ? cos32_q(j) : j <= 16 ? -cos32_q(16 - j) : j <= 24 ? -cos32_q(j - 16) : cos32_q(32 - j);
+}
+__host__ __device__ constexpr double sin32(int j) { return cos32(8 - j); }
+
+// v * exp(D * 2*pi*i * M / R), D = -1 forward, +1 inverse; R divides 32.
+template <int R, int M, int D, typename T>
+__device__ __forceinline__ cplx<T> twc(cplx<T> v) {
+    constexpr int J = ((M * (32 / R)) % 32 + 32) % 32;
+    if constexpr (J == 0) return v;
+    else if constexpr (J == 16) return {-v.x, -v.y};
+    else if constexpr (J == 8) return cmul_i<D>(v);
+    else if constexpr (J == 24) return cmul_i<-D>(v);
+    else {
+        constexpr T c = (T)cos32(J);
+        constexpr T s = (T)(D * sin32(J));
+        return {v.x * c - v.y * s, v.x * s + v.y * c};
+    }
+}
+
+// In-register radix-R DFT, natural order in and out.  R = R1*R2 composites are
+// split as n = R2*n1 + n2, k = k1 + R1*k2 (decimation in time).
+template <int R, int D, typename T>
+__device__ __forceinline__ void dft(cplx<T>* v) {
+    if constexpr (R == 1) {
+        return;
+    } else if constexpr (R == 2) {
+        cplx<T> a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    } else if constexpr (R == 4) {
+        cplx<T> t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+        cplx<T> t2 = cadd(v[1], v[3]), t3 = cmul_i<D>(csub(v[1], v[3]));
+        v[0] = cadd(t0, t2);
+        v[2] = csub(t0, t2);
+        v[1] = cadd(t1, t3);
+        v[3] = csub(t1, t3);
+    } else {
+        constexpr int R1 = 4, R2 = R / 4;
+        static_assert(R % 4 == 0, "radix must be 2, 4 or a multiple of 4");
+        cplx<T> y[R];
+        static_for<0, R2>([&](auto n2c) {
+            constexpr int n2 = decltype(n2c)::value;
+            cplx<T> t[R1];
+            static_for<0, R1>([&](auto n1c) {
+                constexpr int n1 = decltype(n1c)::value;
+                t[n1] = v[R2 * n1 + n2];
+            });
+            dft<R1, D>(t);
+            static_for<0, R1>([&](auto k1c) {
+                constexpr int k1 = decltype(k1c)::value;
+                y[k1 * R2 + n2] = twc<R, n2 * k1, D>(t[k1]);
+            });
+        });
+        static_for<0, R1>([&](auto k1c) {
+            constexpr int k1 = decltype(k1c)::value;
+            cplx<T> t[R2];
+            static_for<0, R2>([&](auto n2c) {
+                constexpr int n2 = decltype(n2c)::value;
+                t[n2] = y[k1 * R2 + n2];
+            });
+            dft<R2, D>(t);
+            static_for<0, R2>([&](auto k2c) {
+                constexpr int k2 = decltype(k2c)::value;
+                v[k1 + R1 * k2] = t[k2];
+            });
+        });
+    }
+}
+
+// LDS index with one pad slot every 32 elements (spreads the strided Stockham
+// stores over the 64 LDS banks).
+__device__ __forceinline__ int lpad(int i) { return i + (i >> 5); }
+template <int N> constexpr int lds_len() { return N + N / 32; }
+
+// Team FFT of length N, P points per thread (TT = N/P threads), radix <= RMAX.
+// D = -1 forward (e^{-i}), +1 inverse; unnormalised.  `tw` holds W_N^k = e^{-2 pi i k/N}.
+// `lds` is the team's private buffer of lds_len<N>() elements.  SYNC is called
+// between the LDS store and load of every exchange and after the load.
+template <typename T, int N, int P, int RMAX, int D>
+struct TeamFFT {
+    static constexpr int TT = N / P;
+    static_assert(N % P == 0, "P must divide N");
+
+    template <int NS, typename SYNC>
+    __device__ static __forceinline__ void stages(cplx<T>* v, cplx<T>* lds, const cplx<T>* __restrict__ tw,
+                                                  int tid, SYNC&& sync) {
+        constexpr int REM = N / NS;
+        constexpr int R0 = REM < RMAX ? REM : RMAX;
+        constexpr int R = R0 < P ? R0 : P;
+        constexpr int Q = P / R;
+        constexpr bool LAST = (NS * R == N);
+        static_assert(P % R == 0, "stage radix must divide P");
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            cplx<T> u[R];
+            const int b = tid + q * TT;
+            const int k = b & (NS - 1);
+#pragma unroll
+            for (int r = 0; r < R; ++r) u[r] = v[q + r * Q];
+            if constexpr (NS > 1) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    cplx<T> w = tw[r * k * (N / (NS * R))];
+                    if constexpr (D > 0) w.y = -w.y;
+                    u[r] = cmul(u[r], w);
+                }
+            }
+            dft<R, D>(u);
+            if constexpr (LAST) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[q + r * Q] = u[r];
+            } else {
+                const int base = (b / NS) * NS * R + k;
+#pragma unroll
+                for (int r = 0; r < R; ++r) lds[lpad(base + r * NS)] = u[r];
+            }
+        }
+        if constexpr (!LAST) {
+            sync();
+#pragma unroll
+            for (int m = 0; m < P; ++m) v[m] = lds[lpad(tid + m * TT)];
+            sync();
+            stages<NS * R>(v, lds, tw, tid, sync);
+        }
+    }
+
+    template <typename SYNC>
+    __device__ static __forceinline__ void run(cplx<T>* v, cplx<T>* lds, const cplx<T>* __restrict__ tw, int tid,
+                                               SYNC&& sync) {
+        stages<1>(v, lds, tw, tid, sync);
+    }
+};
+
+// Team geometry for a transform length: P points per thread, TT threads.
+template <int N>
+struct TeamShape {
+    static constexpr int P = N / 64 > 32 ? 32 : (N / 64 < 8 ? (N < 8 ? N : 8) : N / 64);
+    static constexpr int TT = N / P;
+    static constexpr int RMAX = 16;
+};
+
+}  // namespace shrt

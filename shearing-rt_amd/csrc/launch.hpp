@@ -1,0 +1,32 @@
+// launch.hpp -- host-side launchers of the device kernels, one explicit
+// instantiation per precision (shrt_kern.hip is compiled once for float and
+// once for double).  Each launcher picks the kernel instance for the transform
+// length and returns the launch status.
+#pragma once
+
+#include "aux_kernels.hpp"
+#include "step_kernels.hpp"
+
+namespace shrt {
+
+template <typename T>
+struct Launch {
+    static bool supported(int n);
+    static int team_threads(int n);
+    static hipError_t p1(const StepArgs<T>& a, int grid, hipStream_t s);
+    static hipError_t p2(const StepArgs<T>& a, int grid, hipStream_t s);
+    static hipError_t p3(const StepArgs<T>& a, int grid, hipStream_t s);
+    static hipError_t xfwd(const AuxArgs<T>& a, hipStream_t s);
+    static hipError_t yfwd(const AuxArgs<T>& a, hipStream_t s);
+    static hipError_t yinv(const AuxArgs<T>& a, hipStream_t s);
+    static hipError_t xinv(const AuxArgs<T>& a, hipStream_t s);
+    static hipError_t view_copy(const Geo& g, const void* in, void* out, const int* jmp, hipStream_t s);
+    static hipError_t phi_from_omega(const Geo& g, const void* om, void* phi, const double* kx,
+                                     const double* ky, const double* sh, hipStream_t s);
+    static hipError_t fill_zero(void* p, long long n, hipStream_t s);
+    static hipError_t diag(const Geo& g, const void* om, const void* rh, const int* jmp, const double* kx,
+                           const double* ky, const double* sh, double* partial, int nblocks, double* out,
+                           hipStream_t s);
+};
+
+}  // namespace shrt

@@ -1,0 +1,794 @@
+// shrt_api.hip -- host implementation of the C ABI in include/shrt.h.
+//
+// One context = one instance of the reference's global state (fields.cu:13-25,
+// fft.cu:7-8, shear.cu:21-34), held in HBM and advanced by the three step
+// kernels on the context's own stream.  K steps (params.graph_steps) are
+// captured once into a HIP graph per starting parity and replayed; the device
+// decides the CFL halvings, the shift advance and the remap itself, so no step
+// needs the host.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/shrt.h"
+#include "launch.hpp"
+
+using namespace shrt;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+shrt_status fail(shrt_status st, const std::string& msg) {
+    g_last_error = msg;
+    return st;
+}
+
+#define HIP_TRY(expr)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail(SHRT_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int kHalvingCap = 256;
+constexpr int kDiagBlocks = 512;
+
+}  // namespace
+
+struct shrt_ctx {
+    shrt_params p{};
+    int prec = 4;
+    size_t esz = 8;     // bytes per complex element
+    size_t rsz = 4;     // bytes per real element
+    Geo g{};
+    Phys ph{};
+    hipStream_t stream = nullptr;
+    void* ring[4][3] = {};   // a_omg, d_omg, a_rho, d_rho
+    void* aphi = nullptr;
+    void* G = nullptr;
+    void* H = nullptr;
+    void* tmp_spec = nullptr;
+    void* tmp_spec2 = nullptr;
+    void* tmp_phys = nullptr;
+    double* kx_d = nullptr;
+    double* ky_d = nullptr;
+    void* twx = nullptr;
+    void* twy = nullptr;
+    StepCtl* ctl = nullptr;   // [2]
+    double* s = nullptr;      // [2][nkx]
+    int* jump = nullptr;      // [2][nkx]
+    int* jump_tmp = nullptr;  // [nkx]
+    HalvingRec* hlog = nullptr;
+    double* diag_partial = nullptr;
+    double* diag_out = nullptr;
+    int cur = 0;
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    std::vector<double> kx_h, ky_h;
+    std::vector<void*> allocs;
+};
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// allocation and tables
+// ---------------------------------------------------------------------------
+shrt_status dalloc(shrt_ctx* c, void** p, size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) return fail(SHRT_ERR_ALLOC, std::string("hipMalloc: ") + hipGetErrorString(e));
+    c->allocs.push_back(*p);
+    return SHRT_OK;
+}
+
+template <typename T>
+std::vector<cplx<T>> twiddles(int n) {
+    std::vector<cplx<T>> w(n);
+    for (int k = 0; k < n; ++k) {
+        long double ang = -2.0L * 3.141592653589793238462643383279502884L * (long double)k / (long double)n;
+        w[k] = {(T)cosl(ang), (T)sinl(ang)};
+    }
+    return w;
+}
+
+bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+bool prec_supported(int prec, int n) {
+    return prec == 4 ? Launch<float>::supported(n) : Launch<double>::supported(n);
+}
+
+// ---------------------------------------------------------------------------
+// argument blocks
+// ---------------------------------------------------------------------------
+template <typename T>
+StepArgs<T> step_args(shrt_ctx* c, int in_par, int store_phi) {
+    StepArgs<T> a{};
+    a.g = c->g;
+    a.ph = c->ph;
+    for (int s = 0; s < 3; ++s) {
+        a.a_omg[s] = static_cast<cplx<T>*>(c->ring[0][s]);
+        a.d_omg[s] = static_cast<cplx<T>*>(c->ring[1][s]);
+        a.a_rho[s] = static_cast<cplx<T>*>(c->ring[2][s]);
+        a.d_rho[s] = static_cast<cplx<T>*>(c->ring[3][s]);
+    }
+    a.aphi = static_cast<cplx<T>*>(c->aphi);
+    a.G = static_cast<cplx<T>*>(c->G);
+    a.H = static_cast<cplx<T>*>(c->H);
+    a.kx = c->kx_d;
+    a.ky = c->ky_d;
+    a.twx = static_cast<const cplx<T>*>(c->twx);
+    a.twy = static_cast<const cplx<T>*>(c->twy);
+    a.ctl_in = c->ctl + in_par;
+    a.ctl_out = c->ctl + (in_par ^ 1);
+    a.s_in = c->s + (size_t)in_par * c->g.nkx;
+    a.s_out = c->s + (size_t)(in_par ^ 1) * c->g.nkx;
+    a.jump = c->jump;
+    a.hlog = c->hlog;
+    a.hlog_cap = kHalvingCap;
+    a.store_phi = store_phi;
+    return a;
+}
+
+template <typename T>
+AuxArgs<T> aux_args(shrt_ctx* c, const void* in, void* out, const double* s, const int* jmp, bool xdir, int mode) {
+    AuxArgs<T> a{};
+    a.g = c->g;
+    a.in = in;
+    a.out = out;
+    a.kx = c->kx_d;
+    a.ky = c->ky_d;
+    a.s = s;
+    a.jmp = jmp;
+    a.tw = static_cast<const cplx<T>*>(xdir ? c->twx : c->twy);
+    a.mode = mode;
+    return a;
+}
+
+template <typename T>
+hipError_t enqueue_step_t(shrt_ctx* c, int in_par, int store_phi) {
+    StepArgs<T> a = step_args<T>(c, in_par, store_phi);
+    hipError_t e = Launch<T>::p1(a, c->g.ntile_x * (c->ph.linear ? 1 : 3), c->stream);
+    if (e != hipSuccess) return e;
+    e = Launch<T>::p2(a, c->g.ntile_y, c->stream);
+    if (e != hipSuccess) return e;
+    return Launch<T>::p3(a, c->g.ntile_x, c->stream);
+}
+
+hipError_t enqueue_step(shrt_ctx* c, int in_par, int store_phi) {
+    return c->prec == 4 ? enqueue_step_t<float>(c, in_par, store_phi) : enqueue_step_t<double>(c, in_par, store_phi);
+}
+
+shrt_status build_graph(shrt_ctx* c, int par) {
+    if (c->gexec[par]) return SHRT_OK;
+    const int K = c->p.graph_steps;
+    hipGraph_t graph = nullptr;
+    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t err = hipSuccess;
+    for (int k = 0; k < K && err == hipSuccess; ++k) err = enqueue_step(c, par ^ (k & 1), k == K - 1);
+    hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
+    if (err != hipSuccess) return fail(SHRT_ERR_DEVICE, std::string("capture: ") + hipGetErrorString(err));
+    if (e2 != hipSuccess) return fail(SHRT_ERR_DEVICE, std::string("end capture: ") + hipGetErrorString(e2));
+    hipError_t e3 = hipGraphInstantiate(&c->gexec[par], graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e3 != hipSuccess) {
+        c->gexec[par] = nullptr;
+        return fail(SHRT_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(e3));
+    }
+    return SHRT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// control-block helpers (host side, synchronous)
+// ---------------------------------------------------------------------------
+shrt_status read_ctl(shrt_ctx* c, StepCtl* out) {
+    HIP_TRY(hipMemcpyAsync(out, c->ctl + c->cur, sizeof(StepCtl), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return SHRT_OK;
+}
+
+shrt_status write_ctl(shrt_ctx* c, const StepCtl& v) {
+    HIP_TRY(hipMemcpyAsync(c->ctl + c->cur, &v, sizeof(StepCtl), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return SHRT_OK;
+}
+
+// Remap view of a buffer whose frame is `buf_epoch` at control state `st`.
+shrt_status view_for(shrt_ctx* c, const StepCtl& st, int buf_epoch, const int** jmp) {
+    *jmp = nullptr;
+    if (buf_epoch == st.epoch) return SHRT_OK;
+    if (st.epoch - buf_epoch != 1)
+        return fail(SHRT_ERR_STATE, "buffer is more than one remap behind the current frame");
+    *jmp = c->jump + (size_t)(st.epoch & 1) * c->g.nkx;
+    return SHRT_OK;
+}
+
+hipError_t view_copy(shrt_ctx* c, const void* in, void* out, const int* jmp) {
+    return c->prec == 4 ? Launch<float>::view_copy(c->g, in, out, jmp, c->stream)
+                        : Launch<double>::view_copy(c->g, in, out, jmp, c->stream);
+}
+
+hipError_t phi_from_omega(shrt_ctx* c, const void* om, void* phi, const double* s) {
+    return c->prec == 4 ? Launch<float>::phi_from_omega(c->g, om, phi, c->kx_d, c->ky_d, s, c->stream)
+                        : Launch<double>::phi_from_omega(c->g, om, phi, c->kx_d, c->ky_d, s, c->stream);
+}
+
+size_t spec_bytes(const shrt_ctx* c) { return (size_t)c->g.nkx * c->g.nky * c->esz; }
+size_t phys_bytes(const shrt_ctx* c) { return (size_t)c->g.nx * c->g.ny * c->rsz; }
+
+// Physical slot of the reference's logical ring name (a0/a1/a2 after the last
+// rotation) when `n` steps are complete.
+int logical_slot(long long n, int j) {
+    return j == 0 ? slot_a0(n - 1) : j == 1 ? slot_a1(n - 1) : slot_a2(n - 1);
+}
+
+// xtok on device buffers (phys -> spec), via H field 0 scratch.
+template <typename T>
+hipError_t xtok_dev(shrt_ctx* c, const void* phys, void* spec) {
+    AuxArgs<T> a = aux_args<T>(c, phys, c->H, nullptr, nullptr, true, 0);
+    hipError_t e = Launch<T>::xfwd(a, c->stream);
+    if (e != hipSuccess) return e;
+    AuxArgs<T> b = aux_args<T>(c, c->H, spec, nullptr, nullptr, false, 0);
+    return Launch<T>::yfwd(b, c->stream);
+}
+hipError_t xtok_dev(shrt_ctx* c, const void* phys, void* spec) {
+    return c->prec == 4 ? xtok_dev<float>(c, phys, spec) : xtok_dev<double>(c, phys, spec);
+}
+
+// ktox on device buffers (spec -> phys) with an optional multiplier and view.
+template <typename T>
+hipError_t ktox_dev(shrt_ctx* c, const void* spec, void* phys, const double* s, const int* jmp, int mode) {
+    AuxArgs<T> a = aux_args<T>(c, spec, c->G, s, jmp, false, mode);
+    hipError_t e = Launch<T>::yinv(a, c->stream);
+    if (e != hipSuccess) return e;
+    AuxArgs<T> b = aux_args<T>(c, c->G, phys, s, nullptr, true, 0);
+    return Launch<T>::xinv(b, c->stream);
+}
+hipError_t ktox_dev(shrt_ctx* c, const void* spec, void* phys, const double* s, const int* jmp, int mode) {
+    return c->prec == 4 ? ktox_dev<float>(c, spec, phys, s, jmp, mode) : ktox_dev<double>(c, spec, phys, s, jmp, mode);
+}
+
+hipError_t zero_buffer(shrt_ctx* c, void* p, long long nelem) {
+    return c->prec == 4 ? Launch<float>::fill_zero(p, nelem, c->stream)
+                        : Launch<double>::fill_zero(p, nelem, c->stream);
+}
+
+StepCtl fresh_ctl(const shrt_ctx* c) {
+    StepCtl v{};
+    v.time = 0.0;
+    v.delt = c->p.delt;
+    v.istep = 0;
+    v.epoch = 0;
+    v.last_remap_step = -1;
+    return v;
+}
+
+// Reset every buffer and the control state (the zeroed allocation of
+// fields.cu:120-166 and init_shear, shear.cu:149-185).
+shrt_status reset_state(shrt_ctx* c) {
+    const long long M = (long long)c->g.nkx * c->g.nky;
+    for (int r = 0; r < 4; ++r)
+        for (int s = 0; s < 3; ++s) HIP_TRY(zero_buffer(c, c->ring[r][s], M));
+    HIP_TRY(zero_buffer(c, c->aphi, M));
+    HIP_TRY(hipMemsetAsync(c->s, 0, sizeof(double) * 2 * c->g.nkx, c->stream));
+    HIP_TRY(hipMemsetAsync(c->jump, 0, sizeof(int) * 2 * c->g.nkx, c->stream));
+    StepCtl v = fresh_ctl(c);
+    HIP_TRY(hipMemcpyAsync(c->ctl, &v, sizeof(StepCtl), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->ctl + 1, &v, sizeof(StepCtl), hipMemcpyHostToDevice, c->stream));
+    c->cur = 0;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return SHRT_OK;
+}
+
+uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+// The reference IC on the host in the context precision (time_integral.cu:236-248)
+template <typename T>
+void reference_ic_phys(const shrt_ctx* c, std::vector<T>& omgz, std::vector<T>& rho) {
+    const int nx = c->g.nx, ny = c->g.ny;
+    const T dx = (T)c->p.Lx / (T)nx, dy = (T)c->p.Ly / (T)ny;
+    std::vector<T> xx(nx + 1), yy(ny + 1);
+    for (int ix = 0; ix <= nx; ix++) xx[ix] = ix * dx;
+    for (int iy = 0; iy <= ny; iy++) yy[iy] = iy * dy;
+    omgz.assign((size_t)nx * ny, T(0));
+    rho.assign((size_t)nx * ny, T(0));
+    const T eps1 = (T)c->p.rho_eps1;
+    for (int ix = 0; ix < nx; ix++)
+        for (int iy = 0; iy < ny; iy++) rho[(size_t)ix * ny + iy] = eps1 * std::cos(2 * xx[ix] + 10 * yy[iy]);
+}
+
+// init_dis noise (time_integral.cu:261-307): glibc rand() after srand(2000),
+// ikx-major, two draws per masked mode, added to rho_hat.
+template <typename T>
+void reference_noise(const shrt_ctx* c, std::vector<cplx<T>>& rh) {
+    const int nkx = c->g.nkx, nky = c->g.nky;
+    const T eps2 = (T)(0.1 * (T)c->p.rho_eps1);
+    std::vector<T> kx(nkx), ky(nky);
+    for (int i = 0; i < nkx; ++i) kx[i] = (T)c->kx_h[i];
+    for (int i = 0; i < nky; ++i) ky[i] = (T)c->ky_h[i];
+    srand(2000);
+    for (int ikx = 0; ikx < nkx; ikx++)
+        for (int iky = 0; iky < nky; iky++) {
+            T f1 = 0, f2 = 0;
+            if (std::fabs(kx[ikx]) <= 2 && std::fabs(ky[iky]) != 0 && std::fabs(ky[iky]) <= 10) {
+                f1 = (T)(eps2 * (2 * ((double)rand() / (1.0 + RAND_MAX)) - 1.0));
+                f2 = (T)(eps2 * (2 * ((double)rand() / (1.0 + RAND_MAX)) - 1.0));
+            }
+            cplx<T>& r = rh[(size_t)ikx * nky + iky];
+            r.x = r.x + f1;
+            r.y = r.y + f2;
+        }
+}
+
+template <typename T>
+shrt_status initialize_reference(shrt_ctx* c) {
+    shrt_status st = reset_state(c);
+    if (st != SHRT_OK) return st;
+    std::vector<T> omgz, rho;
+    reference_ic_phys<T>(c, omgz, rho);
+    const int slot = slot_a1(0);
+    HIP_TRY(hipMemcpyAsync(c->tmp_phys, omgz.data(), phys_bytes(c), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(xtok_dev(c, c->tmp_phys, c->ring[0][slot]));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpyAsync(c->tmp_phys, rho.data(), phys_bytes(c), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(xtok_dev(c, c->tmp_phys, c->ring[2][slot]));
+    if (c->p.noise) {
+        std::vector<cplx<T>> rh((size_t)c->g.nkx * c->g.nky);
+        HIP_TRY(hipMemcpyAsync(rh.data(), c->ring[2][slot], spec_bytes(c), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        reference_noise<T>(c, rh);
+        HIP_TRY(hipMemcpyAsync(c->ring[2][slot], rh.data(), spec_bytes(c), hipMemcpyHostToDevice, c->stream));
+    }
+    HIP_TRY(phi_from_omega(c, c->ring[0][slot], c->aphi, c->s));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return SHRT_OK;
+}
+
+template <typename T>
+shrt_status initialize_seeded(shrt_ctx* c, const shrt_seeded_ic* ic) {
+    shrt_status st = reset_state(c);
+    if (st != SHRT_OK) return st;
+    const int nkx = c->g.nkx, nky = c->g.nky, nkxh = c->g.nkxh;
+    std::vector<cplx<T>> rh((size_t)nkx * nky, cplx<T>{T(0), T(0)});
+    for (int ikx = 0; ikx < nkx; ++ikx) {
+        const double kxv = (ikx < nkxh ? ikx : ikx - nkx) * 2 * M_PI / c->p.Lx;
+        for (int iky = 0; iky < nky; ++iky) {
+            if (iky == 0 && ikx >= nkxh) continue;
+            const double kyv = iky * 2 * M_PI / c->p.Ly;
+            const double km = std::sqrt(kxv * kxv + kyv * kyv);
+            if (km < ic->kmin || km > ic->kmax || km == 0.0) continue;
+            const uint64_t z = splitmix64(ic->seed ^ (0x9E3779B97F4A7C15ull * (uint64_t)(ikx * nky + iky + 1)));
+            const double u = (double)(z >> 11) * (1.0 / 9007199254740992.0);
+            const double th = 2 * M_PI * u;
+            const double a = ic->amp * std::pow(km, ic->expo);
+            const double re = a * std::cos(th), im = a * std::sin(th);
+            rh[(size_t)ikx * nky + iky] = {(T)re, (T)im};
+            if (iky == 0 && ikx > 0) rh[(size_t)(nkx - ikx) * nky] = {(T)re, (T)(-im)};
+        }
+    }
+    const int slot = slot_a1(0);
+    HIP_TRY(hipMemcpyAsync(c->ring[2][slot], rh.data(), spec_bytes(c), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return SHRT_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char* shrt_last_error(void) { return g_last_error.c_str(); }
+const char* shrt_version(void) { return "shrt-mi355x 0.1"; }
+
+int shrt_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+shrt_status shrt_sizes(int nx, int ny, int* o) {
+    if (!o || nx < 1 || ny < 1) return fail(SHRT_ERR_INVALID, "shrt_sizes: bad arguments");
+    const int nkx = (nx - 1) / 3 * 2 + 1, nky = (ny - 1) / 3 + 1, nkxh = (nkx + 1) / 2;
+    o[0] = nkx; o[1] = nky; o[2] = nkxh; o[3] = nx - nkxh; o[4] = nx - nkxh * 2 + 1; o[5] = ny / 2 + 1;
+    o[6] = nkx * nky; o[7] = nkxh * ny;
+    return SHRT_OK;
+}
+
+int shrt_supported(int nx, int ny, int precision) {
+    if (precision != SHRT_FP32 && precision != SHRT_FP64) return 0;
+    if (!is_pow2(nx) || !is_pow2(ny)) return 0;
+    return prec_supported(precision, nx) && prec_supported(precision, ny) ? 1 : 0;
+}
+
+shrt_status shrt_create(const shrt_params* p, shrt_ctx** out) {
+    if (!p || !out) return fail(SHRT_ERR_INVALID, "shrt_create: null argument");
+    *out = nullptr;
+    if (p->precision != SHRT_FP32 && p->precision != SHRT_FP64)
+        return fail(SHRT_ERR_INVALID, "precision must be SHRT_FP32 (4) or SHRT_FP64 (8)");
+    if (!shrt_supported(p->nx, p->ny, p->precision))
+        return fail(SHRT_ERR_UNSUPPORTED, "grid " + std::to_string(p->nx) + "x" + std::to_string(p->ny) +
+                                              " is not a supported power-of-two size for this precision");
+    if (p->graph_steps < 0 || (p->graph_steps % 2) != 0)
+        return fail(SHRT_ERR_INVALID, "graph_steps must be 0 or an even number");
+    if (!(p->delt > 0) || !(p->Lx > 0) || !(p->Ly > 0) || p->rho0 == 0)
+        return fail(SHRT_ERR_INVALID, "delt, Lx, Ly must be positive and rho0 nonzero");
+    int sz[8];
+    shrt_sizes(p->nx, p->ny, sz);
+    if (p->sigma != 0 && sz[2] > sz[1])
+        return fail(SHRT_ERR_INVALID, "remap needs nkxh <= nky (nx <= ny) when sigma != 0");
+    if (std::fabs(p->sigma) * p->delt * 3.0 > 1.0)
+        return fail(SHRT_ERR_INVALID, "|sigma|*delt must be below 1/3 (remaps would overlap)");
+
+    shrt_ctx* c = new shrt_ctx();
+    c->p = *p;
+    c->prec = p->precision;
+    c->rsz = c->prec;
+    c->esz = 2 * c->prec;
+    Geo& g = c->g;
+    g.nx = p->nx; g.ny = p->ny;
+    g.nkx = sz[0]; g.nky = sz[1]; g.nkxh = sz[2]; g.nkxh2 = sz[3]; g.nkxpad = sz[4]; g.ncy = sz[5];
+    g.ntile_x = (g.nkxh + kTile - 1) / kTile;
+    g.ntile_y = g.ny / kTile;
+    g.I = (long long)g.nkxh * g.ny;
+    g.Lx = p->Lx; g.Ly = p->Ly;
+    g.dx = p->Lx / p->nx; g.dy = p->Ly / p->ny;
+    g.scale = 1.0 / ((double)p->nx * (double)p->ny);
+    Phys& ph = c->ph;
+    ph.nu = p->nu; ph.kappa = p->kappa; ph.sigma = p->sigma; ph.g = p->g; ph.rho0 = p->rho0;
+    ph.rho0_prime = p->rho0_prime; ph.cfl_num = p->cfl_num; ph.eps = 1e-10; ph.linear = p->linear ? 1 : 0;
+
+    auto bail = [&](shrt_status st) {
+        shrt_destroy(c);
+        return st;
+    };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(SHRT_ERR_DEVICE, "hipStreamCreate failed"));
+    const size_t M = (size_t)g.nkx * g.nky;
+    shrt_status st;
+    for (int r = 0; r < 4; ++r)
+        for (int s = 0; s < 3; ++s)
+            if ((st = dalloc(c, &c->ring[r][s], M * c->esz)) != SHRT_OK) return bail(st);
+    if ((st = dalloc(c, &c->aphi, M * c->esz)) != SHRT_OK) return bail(st);
+    if ((st = dalloc(c, &c->G, 6 * (size_t)g.I * c->esz)) != SHRT_OK) return bail(st);
+    if ((st = dalloc(c, &c->H, 2 * (size_t)g.I * c->esz)) != SHRT_OK) return bail(st);
+    if ((st = dalloc(c, &c->tmp_spec, M * c->esz)) != SHRT_OK) return bail(st);
+    if ((st = dalloc(c, &c->tmp_spec2, M * c->esz)) != SHRT_OK) return bail(st);
+    if ((st = dalloc(c, &c->tmp_phys, (size_t)g.nx * g.ny * c->rsz)) != SHRT_OK) return bail(st);
+    if ((st = dalloc(c, (void**)&c->kx_d, sizeof(double) * g.nkx)) != SHRT_OK) return bail(st);
+    if ((st = dalloc(c, (void**)&c->ky_d, sizeof(double) * g.nky)) != SHRT_OK) return bail(st);
+    if ((st = dalloc(c, &c->twx, (size_t)g.nx * c->esz)) != SHRT_OK) return bail(st);
+    if ((st = dalloc(c, &c->twy, (size_t)g.ny * c->esz)) != SHRT_OK) return bail(st);
+    if ((st = dalloc(c, (void**)&c->ctl, 2 * sizeof(StepCtl))) != SHRT_OK) return bail(st);
+    if ((st = dalloc(c, (void**)&c->s, 2 * sizeof(double) * g.nkx)) != SHRT_OK) return bail(st);
+    if ((st = dalloc(c, (void**)&c->jump, 2 * sizeof(int) * g.nkx)) != SHRT_OK) return bail(st);
+    if ((st = dalloc(c, (void**)&c->jump_tmp, sizeof(int) * g.nkx)) != SHRT_OK) return bail(st);
+    if ((st = dalloc(c, (void**)&c->hlog, sizeof(HalvingRec) * kHalvingCap)) != SHRT_OK) return bail(st);
+    if ((st = dalloc(c, (void**)&c->diag_partial, sizeof(double) * 2 * kDiagQ * kDiagBlocks)) != SHRT_OK)
+        return bail(st);
+    if ((st = dalloc(c, (void**)&c->diag_out, sizeof(double) * 4)) != SHRT_OK) return bail(st);
+
+    // k tables exactly as fourier.cu:123-128 builds them, kept in FP64.
+    c->kx_h.assign(g.nkx, 0.0);
+    for (int ikx = 0; ikx < g.nkxh; ikx++) c->kx_h[ikx] = ikx * 2 * M_PI / p->Lx;
+    for (int ikx = 0; ikx < g.nkxh - 1; ikx++) c->kx_h[g.nkx - ikx - 1] = -c->kx_h[ikx + 1];
+    c->ky_h.assign(g.nky, 0.0);
+    for (int iky = 0; iky < g.nky; iky++) c->ky_h[iky] = iky * 2 * M_PI / p->Ly;
+    auto up = [&](void* d, const void* h, size_t n) { return hipMemcpy(d, h, n, hipMemcpyHostToDevice); };
+    bool ok = up(c->kx_d, c->kx_h.data(), sizeof(double) * g.nkx) == hipSuccess &&
+              up(c->ky_d, c->ky_h.data(), sizeof(double) * g.nky) == hipSuccess;
+    if (c->prec == 4) {
+        auto wx = twiddles<float>(g.nx), wy = twiddles<float>(g.ny);
+        ok = ok && up(c->twx, wx.data(), wx.size() * sizeof(wx[0])) == hipSuccess &&
+             up(c->twy, wy.data(), wy.size() * sizeof(wy[0])) == hipSuccess;
+    } else {
+        auto wx = twiddles<double>(g.nx), wy = twiddles<double>(g.ny);
+        ok = ok && up(c->twx, wx.data(), wx.size() * sizeof(wx[0])) == hipSuccess &&
+             up(c->twy, wy.data(), wy.size() * sizeof(wy[0])) == hipSuccess;
+    }
+    if (!ok) return bail(fail(SHRT_ERR_DEVICE, "table upload failed"));
+    if ((st = reset_state(c)) != SHRT_OK) return bail(st);
+    *out = c;
+    return SHRT_OK;
+}
+
+shrt_status shrt_destroy(shrt_ctx* c) {
+    if (!c) return SHRT_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (int i = 0; i < 2; ++i)
+        if (c->gexec[i]) (void)hipGraphExecDestroy(c->gexec[i]);
+    for (void* p : c->allocs) (void)hipFree(p);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return SHRT_OK;
+}
+
+shrt_status shrt_initialize(shrt_ctx* c) {
+    if (!c) return fail(SHRT_ERR_INVALID, "null context");
+    return c->prec == 4 ? initialize_reference<float>(c) : initialize_reference<double>(c);
+}
+
+shrt_status shrt_initialize_seeded(shrt_ctx* c, const shrt_seeded_ic* ic) {
+    if (!c || !ic) return fail(SHRT_ERR_INVALID, "null argument");
+    return c->prec == 4 ? initialize_seeded<float>(c, ic) : initialize_seeded<double>(c, ic);
+}
+
+shrt_status shrt_step_async(shrt_ctx* c, int nsteps) {
+    if (!c || nsteps < 0) return fail(SHRT_ERR_INVALID, "shrt_step: bad arguments");
+    const int K = c->p.graph_steps;
+    while (nsteps > 0) {
+        if (K >= 2 && nsteps >= K) {
+            shrt_status st = build_graph(c, c->cur);
+            if (st != SHRT_OK) return st;
+            HIP_TRY(hipGraphLaunch(c->gexec[c->cur], c->stream));
+            nsteps -= K;   // K is even: the parity is unchanged
+        } else {
+            HIP_TRY(enqueue_step(c, c->cur, nsteps == 1));
+            c->cur ^= 1;
+            nsteps -= 1;
+        }
+    }
+    return SHRT_OK;
+}
+
+shrt_status shrt_synchronize(shrt_ctx* c) {
+    if (!c) return fail(SHRT_ERR_INVALID, "null context");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    StepCtl v;
+    shrt_status st = read_ctl(c, &v);
+    if (st != SHRT_OK) return st;
+    if (v.error) return fail(SHRT_ERR_STATE, "device reported a remap while a history buffer was still pending");
+    return SHRT_OK;
+}
+
+shrt_status shrt_step(shrt_ctx* c, int nsteps) {
+    shrt_status st = shrt_step_async(c, nsteps);
+    if (st != SHRT_OK) return st;
+    return shrt_synchronize(c);
+}
+
+shrt_status shrt_get_time(shrt_ctx* c, shrt_time* out) {
+    if (!c || !out) return fail(SHRT_ERR_INVALID, "null argument");
+    StepCtl v;
+    shrt_status st = read_ctl(c, &v);
+    if (st != SHRT_OK) return st;
+    out->istep = v.istep;
+    out->time = v.time;
+    out->delt = v.delt;
+    out->remap_count = v.remap_count;
+    out->halvings = v.halvings;
+    out->last_remap_step = v.last_remap_step;
+    return SHRT_OK;
+}
+
+shrt_status shrt_set_time(shrt_ctx* c, int64_t istep, double time, double delt) {
+    if (!c || istep < 0 || !(delt > 0)) return fail(SHRT_ERR_INVALID, "shrt_set_time: bad arguments");
+    StepCtl v;
+    shrt_status st = read_ctl(c, &v);
+    if (st != SHRT_OK) return st;
+    // Keep the slot frames: slots are addressed by istep mod 3, so re-label them.
+    const int e = v.epoch;
+    v.istep = istep;
+    v.time = time;
+    v.delt = delt;
+    for (int s = 0; s < 3; ++s) { v.slot_epoch_a[s] = e; v.slot_epoch_d[s] = e; }
+    v.phi_epoch = e;
+    return write_ctl(c, v);
+}
+
+shrt_status shrt_get_shift(shrt_ctx* c, double* ky_shift, int* jump) {
+    if (!c) return fail(SHRT_ERR_INVALID, "null context");
+    StepCtl v;
+    shrt_status st = read_ctl(c, &v);
+    if (st != SHRT_OK) return st;
+    if (ky_shift)
+        HIP_TRY(hipMemcpyAsync(ky_shift, c->s + (size_t)c->cur * c->g.nkx, sizeof(double) * c->g.nkx,
+                               hipMemcpyDeviceToHost, c->stream));
+    if (jump) {
+        if (v.epoch == 0) memset(jump, 0, sizeof(int) * c->g.nkx);
+        else HIP_TRY(hipMemcpyAsync(jump, c->jump + (size_t)(v.epoch & 1) * c->g.nkx, sizeof(int) * c->g.nkx,
+                                    hipMemcpyDeviceToHost, c->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return SHRT_OK;
+}
+
+shrt_status shrt_set_shift(shrt_ctx* c, const double* ky_shift) {
+    if (!c || !ky_shift) return fail(SHRT_ERR_INVALID, "null argument");
+    HIP_TRY(hipMemcpyAsync(c->s + (size_t)c->cur * c->g.nkx, ky_shift, sizeof(double) * c->g.nkx,
+                           hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return SHRT_OK;
+}
+
+shrt_status shrt_get_spectral(shrt_ctx* c, int field, void* host) {
+    if (!c || !host) return fail(SHRT_ERR_INVALID, "null argument");
+    StepCtl v;
+    shrt_status st = read_ctl(c, &v);
+    if (st != SHRT_OK) return st;
+    const int slot = slot_a1(v.istep);
+    const void* src;
+    int ep;
+    switch (field) {
+        case SHRT_OMEGA: src = c->ring[0][slot]; ep = v.slot_epoch_a[slot]; break;
+        case SHRT_RHO: src = c->ring[2][slot]; ep = v.slot_epoch_a[slot]; break;
+        case SHRT_PHI: src = c->aphi; ep = v.phi_epoch; break;
+        default: return fail(SHRT_ERR_INVALID, "unknown spectral field");
+    }
+    const int* jmp;
+    if ((st = view_for(c, v, ep, &jmp)) != SHRT_OK) return st;
+    HIP_TRY(view_copy(c, src, c->tmp_spec, jmp));
+    HIP_TRY(hipMemcpyAsync(host, c->tmp_spec, spec_bytes(c), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return SHRT_OK;
+}
+
+shrt_status shrt_set_spectral(shrt_ctx* c, int field, const void* host) {
+    if (!c || !host) return fail(SHRT_ERR_INVALID, "null argument");
+    StepCtl v;
+    shrt_status st = read_ctl(c, &v);
+    if (st != SHRT_OK) return st;
+    const int slot = slot_a1(v.istep);
+    void* dst;
+    switch (field) {
+        case SHRT_OMEGA: dst = c->ring[0][slot]; break;
+        case SHRT_RHO: dst = c->ring[2][slot]; break;
+        case SHRT_PHI: dst = c->aphi; break;
+        default: return fail(SHRT_ERR_INVALID, "unknown spectral field");
+    }
+    HIP_TRY(hipMemcpyAsync(dst, host, spec_bytes(c), hipMemcpyHostToDevice, c->stream));
+    if (field == SHRT_OMEGA) {
+        HIP_TRY(phi_from_omega(c, dst, c->aphi, c->s + (size_t)c->cur * c->g.nkx));
+        v.phi_epoch = v.epoch;
+    }
+    if (field == SHRT_PHI) v.phi_epoch = v.epoch;
+    else v.slot_epoch_a[slot] = v.epoch;
+    return write_ctl(c, v);
+}
+
+shrt_status shrt_get_buffer(shrt_ctx* c, int ring, int j, void* host) {
+    if (!c || !host || ring < 0 || ring > 3 || j < 0 || j > 2) return fail(SHRT_ERR_INVALID, "bad arguments");
+    StepCtl v;
+    shrt_status st = read_ctl(c, &v);
+    if (st != SHRT_OK) return st;
+    const int slot = logical_slot(v.istep, j);
+    const int ep = (ring == 0 || ring == 2) ? v.slot_epoch_a[slot] : v.slot_epoch_d[slot];
+    const int* jmp;
+    if ((st = view_for(c, v, ep, &jmp)) != SHRT_OK) return st;
+    HIP_TRY(view_copy(c, c->ring[ring][slot], c->tmp_spec, jmp));
+    HIP_TRY(hipMemcpyAsync(host, c->tmp_spec, spec_bytes(c), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return SHRT_OK;
+}
+
+shrt_status shrt_set_buffer(shrt_ctx* c, int ring, int j, const void* host) {
+    if (!c || !host || ring < 0 || ring > 3 || j < 0 || j > 2) return fail(SHRT_ERR_INVALID, "bad arguments");
+    StepCtl v;
+    shrt_status st = read_ctl(c, &v);
+    if (st != SHRT_OK) return st;
+    const int slot = logical_slot(v.istep, j);
+    HIP_TRY(hipMemcpyAsync(c->ring[ring][slot], host, spec_bytes(c), hipMemcpyHostToDevice, c->stream));
+    if (ring == 0 || ring == 2) v.slot_epoch_a[slot] = v.epoch;
+    else v.slot_epoch_d[slot] = v.epoch;
+    return write_ctl(c, v);
+}
+
+shrt_status shrt_get_physical(shrt_ctx* c, int pf, int lab_frame, void* host) {
+    if (!c || !host) return fail(SHRT_ERR_INVALID, "null argument");
+    if (lab_frame) return fail(SHRT_ERR_UNSUPPORTED, "lab-frame output (ktox_shear) is not implemented");
+    StepCtl v;
+    shrt_status st = read_ctl(c, &v);
+    if (st != SHRT_OK) return st;
+    const int slot = slot_a1(v.istep);
+    const double* s_now = c->s + (size_t)c->cur * c->g.nkx;
+    const double* s_phi = v.istep > 0 ? c->s + (size_t)(c->cur ^ 1) * c->g.nkx : s_now;
+    const void* src = nullptr;
+    int mode = 0, ep = v.epoch;
+    const double* s = s_now;
+    switch (pf) {
+        case SHRT_P_OMEGA: src = c->ring[0][slot]; ep = v.slot_epoch_a[slot]; break;
+        case SHRT_P_RHO: src = c->ring[2][slot]; ep = v.slot_epoch_a[slot]; break;
+        case SHRT_P_PHI: src = c->aphi; ep = v.phi_epoch; break;
+        case SHRT_P_VX: src = c->aphi; mode = 1; s = s_phi; break;   // raw aphi of the last CFL check
+        case SHRT_P_VY: src = c->aphi; mode = 2; s = s_phi; break;
+        default: return fail(SHRT_ERR_INVALID, "unknown physical field");
+    }
+    const int* jmp = nullptr;
+    if (mode == 0 && (st = view_for(c, v, ep, &jmp)) != SHRT_OK) return st;
+    HIP_TRY(ktox_dev(c, src, c->tmp_phys, s, jmp, mode));
+    HIP_TRY(hipMemcpyAsync(host, c->tmp_phys, phys_bytes(c), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return SHRT_OK;
+}
+
+shrt_status shrt_xtok(shrt_ctx* c, const void* phys_in, void* spec_out) {
+    if (!c || !phys_in || !spec_out) return fail(SHRT_ERR_INVALID, "null argument");
+    HIP_TRY(hipMemcpyAsync(c->tmp_phys, phys_in, phys_bytes(c), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(xtok_dev(c, c->tmp_phys, c->tmp_spec));
+    HIP_TRY(hipMemcpyAsync(spec_out, c->tmp_spec, spec_bytes(c), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return SHRT_OK;
+}
+
+shrt_status shrt_ktox(shrt_ctx* c, const void* spec_in, void* phys_out) {
+    if (!c || !spec_in || !phys_out) return fail(SHRT_ERR_INVALID, "null argument");
+    HIP_TRY(hipMemcpyAsync(c->tmp_spec, spec_in, spec_bytes(c), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(ktox_dev(c, c->tmp_spec, c->tmp_phys, c->s + (size_t)c->cur * c->g.nkx, nullptr, 0));
+    HIP_TRY(hipMemcpyAsync(phys_out, c->tmp_phys, phys_bytes(c), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return SHRT_OK;
+}
+
+shrt_status shrt_shearing_field(shrt_ctx* c, const int* jump, const void* in, void* out) {
+    if (!c || !jump || !in || !out) return fail(SHRT_ERR_INVALID, "null argument");
+    for (int i = 0; i < c->g.nkx; ++i) {
+        const int j = jump[i];
+        if (j > c->g.nky + c->g.nkx || j < -(c->g.nky + c->g.nkx))
+            return fail(SHRT_ERR_INVALID, "jump out of range");
+    }
+    HIP_TRY(hipMemcpyAsync(c->jump_tmp, jump, sizeof(int) * c->g.nkx, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->tmp_spec, in, spec_bytes(c), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(view_copy(c, c->tmp_spec, c->tmp_spec2, c->jump_tmp));
+    HIP_TRY(hipMemcpyAsync(out, c->tmp_spec2, spec_bytes(c), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return SHRT_OK;
+}
+
+shrt_status shrt_diagnostics(shrt_ctx* c, shrt_diag* out) {
+    if (!c || !out) return fail(SHRT_ERR_INVALID, "null argument");
+    StepCtl v;
+    shrt_status st = read_ctl(c, &v);
+    if (st != SHRT_OK) return st;
+    const int slot = slot_a1(v.istep);
+    const int* jmp;
+    if ((st = view_for(c, v, v.slot_epoch_a[slot], &jmp)) != SHRT_OK) return st;
+    const double* s = c->s + (size_t)c->cur * c->g.nkx;
+    hipError_t e = c->prec == 4
+        ? Launch<float>::diag(c->g, c->ring[0][slot], c->ring[2][slot], jmp, c->kx_d, c->ky_d, s, c->diag_partial,
+                              kDiagBlocks, c->diag_out, c->stream)
+        : Launch<double>::diag(c->g, c->ring[0][slot], c->ring[2][slot], jmp, c->kx_d, c->ky_d, s, c->diag_partial,
+                               kDiagBlocks, c->diag_out, c->stream);
+    HIP_TRY(e);
+    double h[4];
+    HIP_TRY(hipMemcpyAsync(h, c->diag_out, sizeof(double) * 3, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    out->energy = h[0];
+    out->enstrophy = h[1];
+    out->rho_var = h[2];
+    return SHRT_OK;
+}
+
+shrt_status shrt_get_halvings(shrt_ctx* c, shrt_halving* out, int cap, int* count) {
+    if (!c || !count || cap < 0) return fail(SHRT_ERR_INVALID, "bad arguments");
+    StepCtl v;
+    shrt_status st = read_ctl(c, &v);
+    if (st != SHRT_OK) return st;
+    int n = v.log_count < cap ? v.log_count : cap;
+    std::vector<HalvingRec> h(n > 0 ? n : 1);
+    if (n > 0) {
+        HIP_TRY(hipMemcpyAsync(h.data(), c->hlog, sizeof(HalvingRec) * n, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    for (int i = 0; i < n && out; ++i) {
+        out[i].istep = h[i].istep;
+        out[i].time = h[i].time;
+        out[i].value = h[i].value;
+        out[i].delt = h[i].delt;
+        out[i].which = h[i].which;
+        out[i].count = h[i].pad_;
+    }
+    *count = n;
+    return SHRT_OK;
+}
+
+}  // extern "C"

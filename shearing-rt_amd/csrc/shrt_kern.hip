@@ -1,0 +1,163 @@
+// shrt_kern.hip -- kernel instances and launchers for one precision.
+// Built twice: -DSHRT_REAL=float and -DSHRT_REAL=double.
+#include "launch.hpp"
+
+#ifndef SHRT_REAL
+#error "SHRT_REAL must be float or double"
+#endif
+
+namespace shrt {
+
+namespace {
+
+using R = SHRT_REAL;
+
+// Transform lengths compiled in.  A length fits when the four teams' LDS
+// buffers fit the 160 KiB of a CDNA4 workgroup.
+template <int N>
+constexpr bool fits() {
+    return (long long)kTile * lds_len<N>() * (long long)sizeof(cplx<R>) <= 160 * 1024;
+}
+
+template <typename F>
+bool dispatch(int n, F&& f) {
+    switch (n) {
+#define SHRT_CASE(N)                                              \
+    case N:                                                       \
+        if constexpr (fits<N>()) {                                \
+            f(std::integral_constant<int, N>{});                  \
+            return true;                                          \
+        } else {                                                  \
+            return false;                                         \
+        }
+        SHRT_CASE(32)
+        SHRT_CASE(64)
+        SHRT_CASE(128)
+        SHRT_CASE(256)
+        SHRT_CASE(512)
+        SHRT_CASE(1024)
+        SHRT_CASE(2048)
+        SHRT_CASE(4096)
+#undef SHRT_CASE
+        default:
+            return false;
+    }
+}
+
+hipError_t unsupported() { return hipErrorInvalidValue; }
+
+}  // namespace
+
+template <>
+bool Launch<R>::supported(int n) {
+    return dispatch(n, [](auto) {});
+}
+
+template <>
+int Launch<R>::team_threads(int n) {
+    int tt = 0;
+    dispatch(n, [&](auto nc) { tt = TeamShape<decltype(nc)::value>::TT; });
+    return tt;
+}
+
+template <>
+hipError_t Launch<R>::p1(const StepArgs<R>& a, int grid, hipStream_t s) {
+    bool ok = dispatch(a.g.ny, [&](auto nc) {
+        constexpr int N = decltype(nc)::value;
+        hipLaunchKernelGGL((k_p1<R, N>), dim3(grid), dim3(kTile * TeamShape<N>::TT), 0, s, a);
+    });
+    return ok ? hipGetLastError() : unsupported();
+}
+
+template <>
+hipError_t Launch<R>::p2(const StepArgs<R>& a, int grid, hipStream_t s) {
+    bool ok = dispatch(a.g.nx, [&](auto nc) {
+        constexpr int N = decltype(nc)::value;
+        hipLaunchKernelGGL((k_p2<R, N>), dim3(grid), dim3(kTile * TeamShape<N>::TT), 0, s, a);
+    });
+    return ok ? hipGetLastError() : unsupported();
+}
+
+template <>
+hipError_t Launch<R>::p3(const StepArgs<R>& a, int grid, hipStream_t s) {
+    bool ok = dispatch(a.g.ny, [&](auto nc) {
+        constexpr int N = decltype(nc)::value;
+        hipLaunchKernelGGL((k_p3<R, N>), dim3(grid), dim3(kTile * TeamShape<N>::TT), 0, s, a);
+    });
+    return ok ? hipGetLastError() : unsupported();
+}
+
+template <>
+hipError_t Launch<R>::xfwd(const AuxArgs<R>& a, hipStream_t s) {
+    bool ok = dispatch(a.g.nx, [&](auto nc) {
+        constexpr int N = decltype(nc)::value;
+        hipLaunchKernelGGL((k_aux_xfwd<R, N>), dim3(a.g.ntile_y), dim3(kTile * TeamShape<N>::TT), 0, s, a);
+    });
+    return ok ? hipGetLastError() : unsupported();
+}
+
+template <>
+hipError_t Launch<R>::yfwd(const AuxArgs<R>& a, hipStream_t s) {
+    bool ok = dispatch(a.g.ny, [&](auto nc) {
+        constexpr int N = decltype(nc)::value;
+        hipLaunchKernelGGL((k_aux_yfwd<R, N>), dim3(a.g.ntile_x), dim3(kTile * TeamShape<N>::TT), 0, s, a);
+    });
+    return ok ? hipGetLastError() : unsupported();
+}
+
+template <>
+hipError_t Launch<R>::yinv(const AuxArgs<R>& a, hipStream_t s) {
+    bool ok = dispatch(a.g.ny, [&](auto nc) {
+        constexpr int N = decltype(nc)::value;
+        hipLaunchKernelGGL((k_aux_yinv<R, N>), dim3(a.g.ntile_x), dim3(kTile * TeamShape<N>::TT), 0, s, a);
+    });
+    return ok ? hipGetLastError() : unsupported();
+}
+
+template <>
+hipError_t Launch<R>::xinv(const AuxArgs<R>& a, hipStream_t s) {
+    bool ok = dispatch(a.g.nx, [&](auto nc) {
+        constexpr int N = decltype(nc)::value;
+        hipLaunchKernelGGL((k_aux_xinv<R, N>), dim3(a.g.ntile_y), dim3(kTile * TeamShape<N>::TT), 0, s, a);
+    });
+    return ok ? hipGetLastError() : unsupported();
+}
+
+template <>
+hipError_t Launch<R>::view_copy(const Geo& g, const void* in, void* out, const int* jmp, hipStream_t s) {
+    const long long M = (long long)g.nkx * g.nky;
+    hipLaunchKernelGGL((k_view_copy<R>), dim3((unsigned)((M + 255) / 256)), dim3(256), 0, s, g,
+                       static_cast<const cplx<R>*>(in), static_cast<cplx<R>*>(out), jmp);
+    return hipGetLastError();
+}
+
+template <>
+hipError_t Launch<R>::phi_from_omega(const Geo& g, const void* om, void* phi, const double* kx, const double* ky,
+                                     const double* sh, hipStream_t s) {
+    const long long M = (long long)g.nkx * g.nky;
+    hipLaunchKernelGGL((k_phi_from_omega<R>), dim3((unsigned)((M + 255) / 256)), dim3(256), 0, s, g,
+                       static_cast<const cplx<R>*>(om), static_cast<cplx<R>*>(phi), kx, ky, sh);
+    return hipGetLastError();
+}
+
+template <>
+hipError_t Launch<R>::fill_zero(void* p, long long n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_fill_zero<R>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                       static_cast<cplx<R>*>(p), n);
+    return hipGetLastError();
+}
+
+template <>
+hipError_t Launch<R>::diag(const Geo& g, const void* om, const void* rh, const int* jmp, const double* kx,
+                           const double* ky, const double* sh, double* partial, int nblocks, double* out,
+                           hipStream_t s) {
+    hipLaunchKernelGGL((k_diag_partial<R>), dim3(nblocks), dim3(kDiagThreads), 0, s, g,
+                       static_cast<const cplx<R>*>(om), static_cast<const cplx<R>*>(rh), jmp, kx, ky, sh, partial);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_diag_final<R>), dim3(1), dim3(64), 0, s, partial, nblocks, out);
+    return hipGetLastError();
+}
+
+}  // namespace shrt

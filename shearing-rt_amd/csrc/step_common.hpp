@@ -1,0 +1,143 @@
+// step_common.hpp -- data layout, control block and pointwise helpers shared by
+// the step kernels and the auxiliary transforms.
+//
+// HBM layout (one instance):
+//   state rings  a_omg[3], d_omg[3], a_rho[3], d_rho[3], aphi : nkx*nky complex,
+//                reference layout f[ikx*nky + iky] (fields.cu:137-165)
+//   G            6 fields x I complex, I = nkxh*ny; G[f][t][kx][c] holds the
+//                y-inverse transform at (kx >= 0, y = 4t + c)
+//   H            2 fields x I complex, same tiling, the x-forward transform of
+//                the two brackets at (kx >= 0, y)
+//   kx, ky       double tables (fourier.cu:123-128)
+//   s[2][nkx]    FP64 shearing shift ky_shift (shear.cu:282), by step parity
+//   jump[2][nkx] remap offsets of the last two remap events, by epoch parity
+//   ctl[2]       StepCtl, by step parity
+#pragma once
+
+#include "fft.hpp"
+
+namespace shrt {
+
+constexpr int kTile = 4;   // rows (kx) per P1/P3 workgroup == columns (y) per P2 workgroup
+
+struct Geo {
+    int nx, ny, nkx, nky, nkxh, nkxh2, nkxpad, ncy;
+    int ntile_x;   // ceil(nkxh / kTile): row-pair tiles
+    int ntile_y;   // ny / kTile: column tiles
+    long long I;   // nkxh * ny
+    double Lx, Ly, dx, dy;
+    double scale;  // 1 / (nx*ny)  (fft.cu:165)
+};
+
+struct Phys {
+    double nu, kappa, sigma, g, rho0, rho0_prime, cfl_num, eps;
+    int linear;
+    int pad_;
+};
+
+// Device-resident control block.  Step n reads ctl_in and its last kernel
+// writes ctl_out; the two alternate, so one captured graph serves every step.
+struct StepCtl {
+    double time;                       // t_n
+    double delt;                       // delt before this step's CFL check
+    long long istep;                   // n
+    int epoch;                         // number of remaps before step n (frame of step n)
+    int slot_epoch_a[3];               // frame of each a-ring slot (omega and rho)
+    int slot_epoch_d[3];               // frame of each d-ring slot
+    int phi_epoch;                     // frame of aphi
+    int remap_count;
+    int halvings;
+    int error;                         // bit 0: remap while a live slot was still pending
+    int log_count;
+    long long last_remap_step;
+    unsigned long long cfl_bits[2];    // max|vx + sigma*y|, max|vy| as double bits (P2 atomics)
+};
+
+struct HalvingRec {
+    long long istep;
+    double time, value, delt;
+    int which, pad_;
+};
+
+template <typename T>
+struct StepArgs {
+    Geo g;
+    Phys ph;
+    cplx<T>* a_omg[3];
+    cplx<T>* d_omg[3];
+    cplx<T>* a_rho[3];
+    cplx<T>* d_rho[3];
+    cplx<T>* aphi;
+    cplx<T>* G;
+    cplx<T>* H;
+    const double* kx;
+    const double* ky;
+    const cplx<T>* twx;
+    const cplx<T>* twy;
+    StepCtl* ctl_in;
+    StepCtl* ctl_out;
+    const double* s_in;
+    double* s_out;
+    int* jump;
+    HalvingRec* hlog;
+    int hlog_cap;
+    int store_phi;
+};
+
+// Ring slots after the renew_fields rotation of step n (time_integral.cu:474-484):
+// a0 (written) = (-(n+1)) mod 3, a1 (state n) = (-n) mod 3, a2 = (1-n) mod 3.
+__host__ __device__ inline int slot_a0(long long n) { return (int)(((-(n + 1)) % 3 + 3) % 3); }
+__host__ __device__ inline int slot_a1(long long n) { return (int)(((-n) % 3 + 3) % 3); }
+__host__ __device__ inline int slot_a2(long long n) { return (int)(((1 - n) % 3 + 3) % 3); }
+
+template <typename T> __device__ __forceinline__ cplx<T> czero() { return {T(0), T(0)}; }
+
+// Remapped read of a spectral buffer (shear.cu:296-322).  `jmp` is null when the
+// buffer is already in the current frame.
+template <typename T>
+__device__ __forceinline__ cplx<T> view_load(const cplx<T>* __restrict__ buf, int row, int ky, const Geo& g,
+                                             const int* __restrict__ jmp) {
+    if (jmp == nullptr) return buf[(long long)row * g.nky + ky];
+    const int j = jmp[row];
+    if (j <= 0) {
+        const int src = ky - j;
+        return src < g.nky ? buf[(long long)row * g.nky + src] : czero<T>();
+    }
+    if (ky - j > 0) return buf[(long long)row * g.nky + (ky - j)];
+    const int src = j - ky;
+    const int mrow = row == 0 ? 0 : g.nkx - row;   // ikx_indexed (shear.cu:160-162)
+    if (src >= g.nky) return czero<T>();
+    const cplx<T> v = buf[(long long)mrow * g.nky + src];
+    return {v.x, -v.y};
+}
+
+// Sheared ky and kperp2 in FP64 without contraction (shear.cu:335-336).
+__device__ __forceinline__ double kys_of(const double* ky, const double* s, int row, int iky) {
+    return __dadd_rn(ky[iky], s[row]);
+}
+__device__ __forceinline__ double k2_of(double kx, double kys) {
+    return __dadd_rn(__dmul_rn(kx, kx), __dmul_rn(kys, kys));
+}
+
+// The step size after the CFL halvings of check_cfl (time_integral.cu:357-383).
+// Returns the halving count; `which` gets the last condition that halved.
+__device__ __forceinline__ int cfl_delt(const StepCtl* c, const Geo& g, const Phys& ph, double* dt_out,
+                                        int* which) {
+    double dt = c->delt;
+    const double cvx = __longlong_as_double((long long)c->cfl_bits[0]);
+    const double cvy = __longlong_as_double((long long)c->cfl_bits[1]);
+    const double cg = ph.g / ph.rho0;
+    const double cr = ph.rho0_prime;
+    int h = 0;
+    *which = -1;
+    while (__ddiv_rn(__dmul_rn(cvx, dt), g.dx) > ph.cfl_num && h < 2200) { dt /= 2.0; ++h; *which = 0; }
+    while (__ddiv_rn(__dmul_rn(cvy, dt), g.dy) > ph.cfl_num && h < 2200) { dt /= 2.0; ++h; *which = 1; }
+    while (__ddiv_rn(__dmul_rn(cg, dt), g.dx) > ph.cfl_num && h < 2200) { dt /= 2.0; ++h; *which = 2; }
+    while (__ddiv_rn(__dmul_rn(cr, dt), g.dx) > ph.cfl_num && h < 2200) { dt /= 2.0; ++h; *which = 3; }
+    *dt_out = dt;
+    return h;
+}
+
+__device__ __forceinline__ void sync_wg() { __syncthreads(); }
+
+}  // namespace shrt

@@ -1,0 +1,456 @@
+// step_kernels.hpp -- the three fused kernels of one pseudospectral step.
+//
+// The reference's time_advance (time_integral.cu:324-341) runs ~50 launches,
+// 8 cuFFT 2-D transforms and 3 blocking device->host reads per step.  Here the
+// step is three kernels and never talks to the host:
+//
+//   P1  per kx>=0 row pair (kx, -kx), y-inverse transforms.  Reads omega_hat^n,
+//       rho_hat^n (through the remap view when a remap is pending), applies the
+//       Poisson solve phi = omega/kperp2_shear (shear.cu:367-374) and the ik
+//       multipliers (fourier.cu:180-198, shear.cu:342-354), and writes six
+//       y-transformed fields G: X(i kys phi), X(-i kx phi), X(i kx w),
+//       X(i kys w), X(i kx r), X(i kys r).
+//   P2  per y column, x-inverse transforms (two real fields per complex FFT),
+//       the brackets vx*fx + vy*fy of omega and rho (shear.cu:415-434), the CFL
+//       maxima max|vx + sigma*y| and max|vy| (time_integral.cu:359-371), and
+//       the x-forward transform of (-B_omega, -B_rho) (fourier.cu:170-176).
+//   P3  per kx>=0 row pair, y-forward transform, truncation and 1/(nx*ny)
+//       scaling (fft.cu:155-166), buoyancy (time_integral.cu:545-554, 608-617),
+//       Euler / AB2 / AB3 in BDF2 form (:621-695) and implicit diffusion
+//       (:747-767); workgroup 0 also does the step's bookkeeping: CFL halving,
+//       FP64 shift advance, remap trigger and jump table (shear.cu:275-294).
+#pragma once
+
+#include "step_common.hpp"
+
+namespace shrt {
+
+// ---------------------------------------------------------------------------
+// P1
+// ---------------------------------------------------------------------------
+// Spectral multiplier of output `o` of group `grp` applied to F at (row, iky).
+template <typename T>
+__device__ __forceinline__ cplx<T> p1_mult(int grp, int o, const double* __restrict__ ky,
+                                           const double* __restrict__ s, int row, int iky, double kxv,
+                                           cplx<T> F) {
+    const double kys = kys_of(ky, s, row, iky);
+    if (grp == 0) {
+        cplx<T> phi;
+        if (row == 0 && iky == 0) {
+            phi = czero<T>();
+        } else {
+            const T k2 = (T)k2_of(kxv, kys);
+            phi = {F.x / k2, F.y / k2};                      // neg_lapinv_shear
+        }
+        if (o == 0) { const T k = (T)kys; return {-k * phi.y, k * phi.x}; }   // ddy_shear -> vx
+        const T k = (T)kxv;
+        return {k * phi.y, -k * phi.x};                                        // neg_ddx -> vy
+    }
+    if (o == 0) { const T k = (T)kxv; return {-k * F.y, k * F.x}; }           // ddx
+    const T k = (T)kys;
+    return {-k * F.y, k * F.x};                                                // ddy_shear
+}
+
+template <typename T, int NY>
+__global__ void __launch_bounds__(kTile * TeamShape<NY>::TT)
+k_p1(StepArgs<T> a) {
+    using S = TeamShape<NY>;
+    constexpr int P = S::P, TT = S::TT, L = lds_len<NY>();
+    __shared__ cplx<T> sm[kTile * L];
+    const Geo& g = a.g;
+    const int team = threadIdx.x / TT, tid = threadIdx.x - team * TT;
+    const int tile = blockIdx.x % g.ntile_x;
+    const int grp = blockIdx.x / g.ntile_x;   // 0: phi -> (vx, vy); 1: omega; 2: rho
+    const int row = tile * kTile + team;
+    const bool act = row < g.nkxh;
+    const int rn = row == 0 ? 0 : g.nkx - row;
+    const StepCtl* c = a.ctl_in;
+    const long long n = c->istep;
+    const int s1 = slot_a1(n);
+    const cplx<T>* src = grp == 2 ? a.a_rho[s1] : a.a_omg[s1];
+    const int* jmp = (c->slot_epoch_a[s1] != c->epoch) ? a.jump + (c->epoch & 1) * g.nkx : nullptr;
+    cplx<T>* lds = sm + team * L;
+    auto sync = [] { __syncthreads(); };
+    const double kxp = act ? a.kx[row] : 0.0;
+    const double kxn = act ? a.kx[rn] : 0.0;
+    const int nrow_tile = min(kTile, g.nkxh - tile * kTile);
+
+#pragma unroll 1
+    for (int o = 0; o < 2; ++o) {
+        cplx<T> v[P];
+#pragma unroll
+        for (int m = 0; m < P; ++m) {
+            const int e = tid + m * TT;
+            cplx<T> val = czero<T>();
+            if (act) {
+                if (e < g.nky) {
+                    val = p1_mult<T>(grp, o, a.ky, a.s_in, row, e, kxp, view_load(src, row, e, g, jmp));
+                    if (e == 0) {
+                        // Hermitian part of the ky=0 column, as the x-first 2-D C2R uses it.
+                        const cplx<T> vn = p1_mult<T>(grp, o, a.ky, a.s_in, rn, 0, kxn, view_load(src, rn, 0, g, jmp));
+                        val = {(val.x + vn.x) * T(0.5), (val.y - vn.y) * T(0.5)};
+                    }
+                } else if (e > NY - g.nky) {
+                    const int iky = NY - e;
+                    const cplx<T> vn = p1_mult<T>(grp, o, a.ky, a.s_in, rn, iky, kxn, view_load(src, rn, iky, g, jmp));
+                    val = {vn.x, -vn.y};
+                }
+            }
+            v[m] = val;
+        }
+        TeamFFT<T, NY, P, S::RMAX, +1>::run(v, lds, a.twy, tid, sync);
+        // stage as [t][team][c] so each 4x4 block is one contiguous run of G
+#pragma unroll
+        for (int m = 0; m < P; ++m) {
+            const int e = tid + m * TT;
+            sm[(e >> 2) * 16 + team * 4 + (e & 3)] = v[m];
+        }
+        __syncthreads();
+        cplx<T>* Gf = a.G + (long long)(grp * 2 + o) * g.I;
+        for (int idx = threadIdx.x; idx < NY * 4; idx += blockDim.x) {
+            const int t = idx >> 4, w = idx & 15;
+            if ((w >> 2) < nrow_tile) Gf[((long long)t * g.nkxh + tile * kTile) * 4 + w] = sm[idx];
+        }
+        __syncthreads();
+    }
+
+    // aphi (shear.cu:367-374) for the rows of this tile, when requested
+    if (grp == 0 && a.store_phi && act) {
+        for (int iky = tid; iky < g.nky; iky += TT) {
+#pragma unroll
+            for (int side = 0; side < 2; ++side) {
+                if (side == 1 && row == 0) break;
+                const int r = side ? rn : row;
+                const double kxv = side ? kxn : kxp;
+                cplx<T> phi = czero<T>();
+                if (!(r == 0 && iky == 0)) {
+                    const cplx<T> F = view_load(src, r, iky, g, jmp);
+                    const T k2 = (T)k2_of(kxv, kys_of(a.ky, a.s_in, r, iky));
+                    phi = {F.x / k2, F.y / k2};
+                }
+                a.aphi[(long long)r * g.nky + iky] = phi;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// P2
+// ---------------------------------------------------------------------------
+template <typename T, int NX>
+__global__ void __launch_bounds__(kTile * TeamShape<NX>::TT)
+k_p2(StepArgs<T> a) {
+    using S = TeamShape<NX>;
+    constexpr int P = S::P, TT = S::TT, L = lds_len<NX>();
+    constexpr int NQ = (NX / 3 + 1 + TT - 1) / TT;   // >= ceil(nkxh / TT)
+    __shared__ cplx<T> sm[kTile * L];
+    const Geo& g = a.g;
+    const int team = threadIdx.x / TT, tid = threadIdx.x - team * TT;
+    const int tile = blockIdx.x;
+    const int y = tile * kTile + team;
+    const int nk = g.nkxh;
+    cplx<T>* lds = sm + team * L;
+    auto sync = [] { __syncthreads(); };
+    const int npairs = a.ph.linear ? 1 : 3;
+
+    T vx[P], vy[P], bw[P], br[P];
+#pragma unroll 1
+    for (int p = 0; p < npairs; ++p) {
+        const cplx<T>* GA = a.G + (long long)(2 * p) * g.I + (long long)tile * nk * 4;
+        const cplx<T>* GB = GA + g.I;
+        for (int idx = threadIdx.x; idx < nk * 4; idx += blockDim.x) {
+            sm[idx] = GA[idx];
+            sm[nk * 4 + idx] = GB[idx];
+        }
+        __syncthreads();
+        cplx<T> v[P];
+#pragma unroll
+        for (int m = 0; m < P; ++m) {
+            const int e = tid + m * TT;
+            cplx<T> z = czero<T>();
+            if (e < nk) {
+                const cplx<T> A = sm[e * 4 + team], B = sm[nk * 4 + e * 4 + team];
+                z = e == 0 ? cplx<T>{A.x, B.x} : cplx<T>{A.x - B.y, A.y + B.x};
+            } else if (e > NX - nk) {
+                const int kk = NX - e;
+                const cplx<T> A = sm[kk * 4 + team], B = sm[nk * 4 + kk * 4 + team];
+                z = {A.x + B.y, B.x - A.y};
+            }
+            v[m] = z;
+        }
+        __syncthreads();
+        TeamFFT<T, NX, P, S::RMAX, +1>::run(v, lds, a.twx, tid, sync);
+        if (p == 0) {
+#pragma unroll
+            for (int m = 0; m < P; ++m) { vx[m] = v[m].x; vy[m] = v[m].y; }
+        } else if (p == 1) {
+#pragma unroll
+            for (int m = 0; m < P; ++m) bw[m] = vx[m] * v[m].x + vy[m] * v[m].y;
+        } else {
+#pragma unroll
+            for (int m = 0; m < P; ++m) br[m] = vx[m] * v[m].x + vy[m] * v[m].y;
+        }
+    }
+
+    // CFL maxima (add_v0 + pl_max_search): max|vx + sigma*yy[y]|, max|vy|
+    {
+        const double sy = __dmul_rn(a.ph.sigma, (double)y * g.dy);
+        double m0 = 0.0, m1 = 0.0;
+#pragma unroll
+        for (int m = 0; m < P; ++m) {
+            m0 = fmax(m0, fabs(__dadd_rn((double)vx[m], sy)));
+            m1 = fmax(m1, fabs((double)vy[m]));
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            m0 = fmax(m0, __shfl_xor(m0, off));
+            m1 = fmax(m1, __shfl_xor(m1, off));
+        }
+        if ((threadIdx.x & 63) == 0) {
+            atomicMax(&a.ctl_in->cfl_bits[0], (unsigned long long)__double_as_longlong(m0));
+            atomicMax(&a.ctl_in->cfl_bits[1], (unsigned long long)__double_as_longlong(m1));
+        }
+    }
+    if (a.ph.linear) return;
+
+    // x-forward of z = -B_omega - i B_rho, then split into the two spectra
+    cplx<T> v[P];
+#pragma unroll
+    for (int m = 0; m < P; ++m) v[m] = {-bw[m], -br[m]};
+    TeamFFT<T, NX, P, S::RMAX, -1>::run(v, lds, a.twx, tid, sync);
+#pragma unroll
+    for (int m = 0; m < P; ++m) lds[tid + m * TT] = v[m];
+    __syncthreads();
+    cplx<T> hw[NQ], hr[NQ];
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+        const int kx = tid + i * TT;
+        if (kx < nk) {
+            const cplx<T> Zk = lds[kx], Zm = lds[(NX - kx) & (NX - 1)];
+            hw[i] = {(Zk.x + Zm.x) * T(0.5), (Zk.y - Zm.y) * T(0.5)};
+            hr[i] = {(Zk.y + Zm.y) * T(0.5), (Zm.x - Zk.x) * T(0.5)};
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+        const int kx = tid + i * TT;
+        if (kx < nk) {
+            sm[kx * 4 + team] = hw[i];
+            sm[nk * 4 + kx * 4 + team] = hr[i];
+        }
+    }
+    __syncthreads();
+    cplx<T>* Hw = a.H + (long long)tile * nk * 4;
+    cplx<T>* Hr = Hw + g.I;
+    for (int idx = threadIdx.x; idx < nk * 4; idx += blockDim.x) {
+        Hw[idx] = sm[idx];
+        Hr[idx] = sm[nk * 4 + idx];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// P3
+// ---------------------------------------------------------------------------
+// Explicit step + implicit diffusion for one mode (time_integral.cu:621-695,
+// 747-767), in the reference's operation order.
+template <typename T>
+__device__ __forceinline__ cplx<T> ab_bdf_update(int ist, bool plain, double dt, double diff, double k2,
+                                                 cplx<T> a1, cplx<T> a2, cplx<T> d0, cplx<T> d1, cplx<T> d2) {
+    cplx<T> r;
+    if (ist == 0) {                                            // eulerf
+        const T c = (T)dt;
+        r = {a1.x + c * d0.x, a1.y + c * d0.y};
+    } else if (ist == 1) {
+        if (plain) {                                           // ab2
+            const T c = (T)dt;
+            const cplx<T> w = {T(1.5) * d0.x - T(0.5) * d1.x, T(1.5) * d0.y - T(0.5) * d1.y};
+            r = {a1.x + c * w.x, a1.y + c * w.y};
+        } else {                                               // ab2, BDF2 form
+            const T c = (T)((2.0 / 3.0) * dt);
+            const cplx<T> u = {(T(4) * a1.x - a2.x) / T(3), (T(4) * a1.y - a2.y) / T(3)};
+            const cplx<T> w = {T(2) * d0.x - d1.x, T(2) * d0.y - d1.y};
+            r = {u.x + c * w.x, u.y + c * w.y};
+        }
+    } else {
+        if (plain) {                                           // ab3
+            const T c = (T)(dt / 12.0);
+            const cplx<T> w = {(T(23) * d0.x - T(16) * d1.x) + T(5) * d2.x,
+                               (T(23) * d0.y - T(16) * d1.y) + T(5) * d2.y};
+            r = {a1.x + c * w.x, a1.y + c * w.y};
+        } else {                                               // ab3, BDF2 form
+            const T c = (T)(dt / 9.0);
+            const cplx<T> u = {(T(4) * a1.x - a2.x) / T(3), (T(4) * a1.y - a2.y) / T(3)};
+            const cplx<T> w = {(T(16) * d0.x - T(14) * d1.x) + T(4) * d2.x,
+                               (T(16) * d0.y - T(14) * d1.y) + T(4) * d2.y};
+            r = {u.x + c * w.x, u.y + c * w.y};
+        }
+    }
+    if (!plain) {                                              // eulerb / bdf2
+        const double cd = ist == 0 ? dt : __dmul_rn(2.0 / 3.0, dt);
+        const T den = (T)__dadd_rn(1.0, __dmul_rn(__dmul_rn(cd, diff), k2));
+        r = {r.x / den, r.y / den};
+    }
+    return r;
+}
+
+// Bookkeeping of step n done by workgroup 0 of P3: the shift advance and remap
+// trigger of update_shear (shear.cu:211-294) in FP64, and the next control block.
+template <typename T>
+__device__ void p3_bookkeeping(const StepArgs<T>& a, double dt, int h, int which) {
+    const Geo& g = a.g;
+    const StepCtl* c = a.ctl_in;
+    const double sdt1 = g.nkx > 1 ? __dmul_rn(__dmul_rn(a.ph.sigma, a.kx[1]), dt) : 0.0;
+    const double s1n = g.nkx > 1 ? __dsub_rn(a.s_in[1], sdt1) : 0.0;
+    const int trig = (g.nkx > 1 && fabs(__dsub_rn(s1n, -a.kx[1])) < 1e-10) ? 1 : 0;
+    const int newE = c->epoch + trig;
+    int* jn = a.jump + (newE & 1) * g.nkx;
+    for (int i = threadIdx.x; i < g.nkx; i += blockDim.x) {
+        double sn = a.s_in[i];
+        if (i > 0) sn = __dsub_rn(a.s_in[i], __dmul_rn(__dmul_rn(a.ph.sigma, a.kx[i]), dt));
+        if (trig) {
+            jn[i] = i == 0 ? 0 : (int)round(sn / a.kx[1]);
+            if (i > 0) sn = 0.0;
+        }
+        a.s_out[i] = sn;
+    }
+    if (threadIdx.x == 0) {
+        const long long n = c->istep;
+        StepCtl o = *c;
+        o.time = __dadd_rn(c->time, dt);
+        o.delt = dt;
+        o.istep = n + 1;
+        o.epoch = newE;
+        const int s0 = slot_a0(n), s1 = slot_a1(n);
+        if (trig && (c->slot_epoch_a[s1] != c->epoch || c->slot_epoch_d[s1] != c->epoch)) o.error |= 1;
+        o.slot_epoch_a[s0] = c->epoch;
+        o.slot_epoch_d[s0] = c->epoch;
+        if (a.store_phi) o.phi_epoch = c->epoch;
+        o.remap_count = c->remap_count + trig;
+        if (trig) o.last_remap_step = n;
+        o.halvings = c->halvings + h;
+        if (h > 0 && a.hlog != nullptr && c->log_count < a.hlog_cap) {
+            const double vals[4] = {__longlong_as_double((long long)c->cfl_bits[0]),
+                                    __longlong_as_double((long long)c->cfl_bits[1]), a.ph.g / a.ph.rho0,
+                                    a.ph.rho0_prime};
+            HalvingRec rec;
+            rec.istep = n;
+            rec.time = c->time;
+            rec.value = which >= 0 ? vals[which] : 0.0;
+            rec.delt = dt;
+            rec.which = which;
+            rec.pad_ = h;
+            a.hlog[c->log_count] = rec;
+            o.log_count = c->log_count + 1;
+        }
+        o.cfl_bits[0] = 0ull;
+        o.cfl_bits[1] = 0ull;
+        *a.ctl_out = o;
+    }
+}
+
+template <typename T, int NY>
+__global__ void __launch_bounds__(kTile * TeamShape<NY>::TT)
+k_p3(StepArgs<T> a) {
+    using S = TeamShape<NY>;
+    constexpr int P = S::P, TT = S::TT, L = lds_len<NY>();
+    __shared__ cplx<T> sm[kTile * L];
+    const Geo& g = a.g;
+    const Phys& ph = a.ph;
+    const int team = threadIdx.x / TT, tid = threadIdx.x - team * TT;
+    const int tile = blockIdx.x;
+    const int row = tile * kTile + team;
+    const bool act = row < g.nkxh;
+    const int rn = row == 0 ? 0 : g.nkx - row;
+    const int nrow_tile = min(kTile, g.nkxh - tile * kTile);
+    const StepCtl* c = a.ctl_in;
+    const long long n = c->istep;
+    cplx<T>* lds = sm + team * L;
+    auto sync = [] { __syncthreads(); };
+
+    double dt;
+    int which;
+    const int h = cfl_delt(c, g, ph, &dt, &which);
+    if (blockIdx.x == 0) p3_bookkeeping(a, dt, h, which);
+
+    const int s0 = slot_a0(n), s1 = slot_a1(n), s2 = slot_a2(n);
+    const int* jbase = a.jump + (c->epoch & 1) * g.nkx;
+    const int* ja1 = c->slot_epoch_a[s1] != c->epoch ? jbase : nullptr;
+    const int* ja2 = c->slot_epoch_a[s2] != c->epoch ? jbase : nullptr;
+    const int* jd1 = c->slot_epoch_d[s1] != c->epoch ? jbase : nullptr;
+    const int* jd2 = c->slot_epoch_d[s2] != c->epoch ? jbase : nullptr;
+    const int ist = n == 0 ? 0 : (n == 1 ? 1 : 2);
+    const T scale = (T)g.scale;
+
+#pragma unroll 1
+    for (int f = 0; f < 2; ++f) {
+        if (!ph.linear) {
+            const cplx<T>* Hf = a.H + (long long)f * g.I;
+            for (int idx = threadIdx.x; idx < NY * 4; idx += blockDim.x) {
+                const int t = idx >> 4, w = idx & 15;
+                sm[idx] = (w >> 2) < nrow_tile ? Hf[((long long)t * g.nkxh + tile * kTile) * 4 + w] : czero<T>();
+            }
+            __syncthreads();
+            cplx<T> v[P];
+#pragma unroll
+            for (int m = 0; m < P; ++m) {
+                const int e = tid + m * TT;
+                v[m] = sm[(e >> 2) * 16 + team * 4 + (e & 3)];
+            }
+            __syncthreads();
+            TeamFFT<T, NY, P, S::RMAX, -1>::run(v, lds, a.twy, tid, sync);
+#pragma unroll
+            for (int m = 0; m < P; ++m) lds[tid + m * TT] = v[m];
+            __syncthreads();
+        }
+        if (act) {
+            cplx<T>* A0 = f ? a.a_rho[s0] : a.a_omg[s0];
+            const cplx<T>* A1 = f ? a.a_rho[s1] : a.a_omg[s1];
+            const cplx<T>* A2 = f ? a.a_rho[s2] : a.a_omg[s2];
+            cplx<T>* D0 = f ? a.d_rho[s0] : a.d_omg[s0];
+            const cplx<T>* D1 = f ? a.d_rho[s1] : a.d_omg[s1];
+            const cplx<T>* D2 = f ? a.d_rho[s2] : a.d_omg[s2];
+            const cplx<T>* X1 = f ? a.a_omg[s1] : a.a_rho[s1];   // the other field at step n
+            const double diff = f ? ph.kappa : ph.nu;
+            const bool plain = diff < ph.eps;
+            for (int iky = tid; iky < g.nky; iky += TT) {
+#pragma unroll
+                for (int side = 0; side < 2; ++side) {
+                    if (side == 1 && row == 0) break;
+                    const int r = side ? rn : row;
+                    cplx<T> nh = czero<T>();
+                    if (!ph.linear) {
+                        const cplx<T> F = side ? cconj(lds[(NY - iky) & (NY - 1)]) : lds[iky];
+                        nh = {F.x * scale, F.y * scale};
+                    }
+                    const double kxv = a.kx[r];
+                    const double k2 = k2_of(kxv, kys_of(a.ky, a.s_in, r, iky));
+                    const cplx<T> x1 = view_load(X1, r, iky, g, ja1);
+                    cplx<T> d0;
+                    if (f == 0) {                               // add_ddxrho
+                        const T cc = (T)__dmul_rn(ph.g / ph.rho0, kxv);
+                        d0 = {nh.x + cc * x1.y, nh.y - cc * x1.x};
+                    } else {                                    // add_ddxphi with phi = omega/kperp2
+                        cplx<T> phi = czero<T>();
+                        if (!(r == 0 && iky == 0)) {
+                            const T k2t = (T)k2;
+                            phi = {x1.x / k2t, x1.y / k2t};
+                        }
+                        const T cc = (T)__dmul_rn(ph.rho0_prime, kxv);
+                        d0 = {nh.x - cc * phi.y, nh.y + cc * phi.x};
+                    }
+                    const cplx<T> a1 = view_load(A1, r, iky, g, ja1);
+                    const cplx<T> a2 = ist >= 1 ? view_load(A2, r, iky, g, ja2) : czero<T>();
+                    const cplx<T> d1 = ist >= 1 ? view_load(D1, r, iky, g, jd1) : czero<T>();
+                    const cplx<T> d2 = ist >= 2 ? view_load(D2, r, iky, g, jd2) : czero<T>();
+                    const long long at = (long long)r * g.nky + iky;
+                    A0[at] = ab_bdf_update<T>(ist, plain, dt, diff, k2, a1, a2, d0, d1, d2);
+                    D0[at] = d0;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace shrt

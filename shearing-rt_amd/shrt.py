@@ -1,0 +1,268 @@
+"""ctypes binding of libshrt_hip.so (include/shrt.h) for tests, bench and tools.
+
+Mirrors the reference's source-level interface (time_integral.h, fft.h, shear.h):
+`Shrt.initialize()` is initialize(), `Shrt.time_advance(n)` is time_advance
+called n times, `xtok` / `ktox` are fft.h's transforms, `shearing_field` is the
+remap of shear.cu.  Arrays are numpy, in the reference layouts:
+spectral (nkx, nky) complex, physical (nx, ny) real.
+
+The library must be built (make -C shearing-rt_amd); there is no fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libshrt_hip.so")
+REPO = os.path.dirname(HERE)
+HEADER = os.path.join(REPO, "include", "shrt.h")
+
+FP32, FP64 = 4, 8
+OMEGA, PHI, RHO = 0, 1, 2
+P_OMEGA, P_PHI, P_RHO, P_VX, P_VY = 0, 1, 2, 3, 4
+RING = {"aomg": 0, "domg": 1, "arho": 2, "drho": 3}
+
+_lib = None
+
+
+class ShrtError(RuntimeError):
+    pass
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int),
+        ("Lx", ctypes.c_double), ("Ly", ctypes.c_double),
+        ("delt", ctypes.c_double), ("cfl_num", ctypes.c_double),
+        ("nu", ctypes.c_double), ("kappa", ctypes.c_double),
+        ("sigma", ctypes.c_double), ("g", ctypes.c_double),
+        ("rho0", ctypes.c_double), ("rho0_prime", ctypes.c_double),
+        ("rho_eps1", ctypes.c_double),
+        ("noise", ctypes.c_int), ("linear", ctypes.c_int),
+        ("precision", ctypes.c_int), ("graph_steps", ctypes.c_int),
+    ]
+
+
+class SeededIC(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_uint64), ("kmin", ctypes.c_double), ("kmax", ctypes.c_double),
+                ("amp", ctypes.c_double), ("expo", ctypes.c_double)]
+
+
+class Time(ctypes.Structure):
+    _fields_ = [("istep", ctypes.c_int64), ("time", ctypes.c_double), ("delt", ctypes.c_double),
+                ("remap_count", ctypes.c_int), ("halvings", ctypes.c_int),
+                ("last_remap_step", ctypes.c_int64)]
+
+
+class Diag(ctypes.Structure):
+    _fields_ = [("energy", ctypes.c_double), ("enstrophy", ctypes.c_double), ("rho_var", ctypes.c_double)]
+
+
+class Halving(ctypes.Structure):
+    _fields_ = [("istep", ctypes.c_int64), ("time", ctypes.c_double), ("value", ctypes.c_double),
+                ("delt", ctypes.c_double), ("which", ctypes.c_int), ("count", ctypes.c_int)]
+
+
+# Every entry point of include/shrt.h with its ctypes signature.
+_VP = ctypes.c_void_p
+_SIGS = {
+    "shrt_create": ([ctypes.POINTER(Params), ctypes.POINTER(_VP)], ctypes.c_int),
+    "shrt_destroy": ([_VP], ctypes.c_int),
+    "shrt_initialize": ([_VP], ctypes.c_int),
+    "shrt_initialize_seeded": ([_VP, ctypes.POINTER(SeededIC)], ctypes.c_int),
+    "shrt_step": ([_VP, ctypes.c_int], ctypes.c_int),
+    "shrt_step_async": ([_VP, ctypes.c_int], ctypes.c_int),
+    "shrt_synchronize": ([_VP], ctypes.c_int),
+    "shrt_get_spectral": ([_VP, ctypes.c_int, _VP], ctypes.c_int),
+    "shrt_set_spectral": ([_VP, ctypes.c_int, _VP], ctypes.c_int),
+    "shrt_get_buffer": ([_VP, ctypes.c_int, ctypes.c_int, _VP], ctypes.c_int),
+    "shrt_set_buffer": ([_VP, ctypes.c_int, ctypes.c_int, _VP], ctypes.c_int),
+    "shrt_get_physical": ([_VP, ctypes.c_int, ctypes.c_int, _VP], ctypes.c_int),
+    "shrt_xtok": ([_VP, _VP, _VP], ctypes.c_int),
+    "shrt_ktox": ([_VP, _VP, _VP], ctypes.c_int),
+    "shrt_shearing_field": ([_VP, _VP, _VP, _VP], ctypes.c_int),
+    "shrt_get_time": ([_VP, ctypes.POINTER(Time)], ctypes.c_int),
+    "shrt_set_time": ([_VP, ctypes.c_int64, ctypes.c_double, ctypes.c_double], ctypes.c_int),
+    "shrt_get_shift": ([_VP, _VP, _VP], ctypes.c_int),
+    "shrt_set_shift": ([_VP, _VP], ctypes.c_int),
+    "shrt_diagnostics": ([_VP, ctypes.POINTER(Diag)], ctypes.c_int),
+    "shrt_get_halvings": ([_VP, ctypes.POINTER(Halving), ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
+                          ctypes.c_int),
+    "shrt_sizes": ([ctypes.c_int, ctypes.c_int, _VP], ctypes.c_int),
+    "shrt_supported": ([ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "shrt_device_count": ([], ctypes.c_int),
+    "shrt_last_error": ([], ctypes.c_char_p),
+    "shrt_version": ([], ctypes.c_char_p),
+}
+
+
+def lib():
+    """Load libshrt_hip.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ShrtError(f"{LIB_PATH} is missing: build it with `make -C shearing-rt_amd`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in _SIGS.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def _check(st):
+    if st != 0:
+        raise ShrtError(f"status {st}: {lib().shrt_last_error().decode()}")
+
+
+def sizes(nx, ny):
+    out = np.zeros(8, dtype=np.int32)
+    _check(lib().shrt_sizes(nx, ny, out.ctypes.data))
+    return dict(zip(["nkx", "nky", "nkxh", "nkxh2", "nkxpad", "ncy", "M", "I"], map(int, out)))
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+class Shrt:
+    """One simulation context (one instance of the reference's global state)."""
+
+    DEFAULTS = dict(nx=512, ny=1024, Lx=np.pi, Ly=np.pi, delt=1e-3, cfl_num=0.1, nu=1e-3,
+                    kappa=1e-5, sigma=1.0, g=1.0, rho0=1.0, rho0_prime=1.0, rho_eps1=1e-2,
+                    noise=1, linear=0, precision=FP64, graph_steps=0)
+
+    def __init__(self, **kw):
+        p = dict(self.DEFAULTS)
+        p.update(kw)
+        self.params = p
+        ints = ("nx", "ny", "noise", "linear", "precision", "graph_steps")
+        self._p = Params(**{k: (int(v) if k in ints else float(v)) for k, v in p.items()})
+        h = ctypes.c_void_p()
+        _check(lib().shrt_create(ctypes.byref(self._p), ctypes.byref(h)))
+        self.h = h
+        self.nx, self.ny = p["nx"], p["ny"]
+        self.s = sizes(self.nx, self.ny)
+        self.real = np.float32 if p["precision"] == FP32 else np.float64
+        self.cplx = np.complex64 if p["precision"] == FP32 else np.complex128
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().shrt_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def shape_k(self):
+        return (self.s["nkx"], self.s["nky"])
+
+    # --- lifecycle ---------------------------------------------------------
+    def initialize(self):
+        _check(lib().shrt_initialize(self.h))
+
+    def initialize_seeded(self, seed, kmin, kmax, amp, expo):
+        ic = SeededIC(seed, kmin, kmax, amp, expo)
+        _check(lib().shrt_initialize_seeded(self.h, ctypes.byref(ic)))
+
+    def time_advance(self, n=1):
+        _check(lib().shrt_step(self.h, int(n)))
+
+    def step_async(self, n):
+        _check(lib().shrt_step_async(self.h, int(n)))
+
+    def synchronize(self):
+        _check(lib().shrt_synchronize(self.h))
+
+    # --- state access ------------------------------------------------------
+    def spectral(self, field):
+        out = np.zeros(self.shape_k, dtype=self.cplx)
+        _check(lib().shrt_get_spectral(self.h, field, _ptr(out)))
+        return out
+
+    def set_spectral(self, field, arr):
+        a = np.ascontiguousarray(arr, dtype=self.cplx).reshape(self.shape_k)
+        _check(lib().shrt_set_spectral(self.h, field, _ptr(a)))
+
+    def buffer(self, ring, slot=0):
+        out = np.zeros(self.shape_k, dtype=self.cplx)
+        _check(lib().shrt_get_buffer(self.h, RING[ring], slot, _ptr(out)))
+        return out
+
+    def set_buffer(self, ring, slot, arr):
+        a = np.ascontiguousarray(arr, dtype=self.cplx).reshape(self.shape_k)
+        _check(lib().shrt_set_buffer(self.h, RING[ring], slot, _ptr(a)))
+
+    def physical(self, pfield, lab_frame=0):
+        out = np.zeros((self.nx, self.ny), dtype=self.real)
+        _check(lib().shrt_get_physical(self.h, pfield, lab_frame, _ptr(out)))
+        return out
+
+    def xtok(self, phys):
+        a = np.ascontiguousarray(phys, dtype=self.real).reshape(self.nx, self.ny)
+        out = np.zeros(self.shape_k, dtype=self.cplx)
+        _check(lib().shrt_xtok(self.h, _ptr(a), _ptr(out)))
+        return out
+
+    def ktox(self, spec):
+        a = np.ascontiguousarray(spec, dtype=self.cplx).reshape(self.shape_k)
+        out = np.zeros((self.nx, self.ny), dtype=self.real)
+        _check(lib().shrt_ktox(self.h, _ptr(a), _ptr(out)))
+        return out
+
+    def shearing_field(self, jump, spec):
+        j = np.ascontiguousarray(jump, dtype=np.int32)
+        a = np.ascontiguousarray(spec, dtype=self.cplx).reshape(self.shape_k)
+        out = np.zeros_like(a)
+        _check(lib().shrt_shearing_field(self.h, _ptr(j), _ptr(a), _ptr(out)))
+        return out
+
+    def state(self):
+        t = Time()
+        _check(lib().shrt_get_time(self.h, ctypes.byref(t)))
+        return dict(istep=t.istep, time=t.time, delt=t.delt, remap_count=t.remap_count,
+                    halvings=t.halvings, last_remap_step=t.last_remap_step)
+
+    def set_time(self, istep, time, delt):
+        _check(lib().shrt_set_time(self.h, int(istep), float(time), float(delt)))
+
+    def shift(self):
+        s = np.zeros(self.s["nkx"])
+        j = np.zeros(self.s["nkx"], dtype=np.int32)
+        _check(lib().shrt_get_shift(self.h, _ptr(s), _ptr(j)))
+        return s, j
+
+    def set_shift(self, s):
+        a = np.ascontiguousarray(s, dtype=np.float64)
+        _check(lib().shrt_set_shift(self.h, _ptr(a)))
+
+    def diagnostics(self):
+        d = Diag()
+        _check(lib().shrt_diagnostics(self.h, ctypes.byref(d)))
+        return dict(energy=d.energy, enstrophy=d.enstrophy, rho_var=d.rho_var)
+
+    def halvings(self, cap=256):
+        arr = (Halving * cap)()
+        n = ctypes.c_int()
+        _check(lib().shrt_get_halvings(self.h, arr, cap, ctypes.byref(n)))
+        return [dict(istep=arr[i].istep, time=arr[i].time, value=arr[i].value, delt=arr[i].delt,
+                     which=arr[i].which, count=arr[i].count) for i in range(n.value)]
+
+
+def header_symbols():
+    """Function names declared in include/shrt.h."""
+    import re
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(shrt_[a-z_0-9]+)\s*\(", txt)))

@@ -1,0 +1,243 @@
+"""GPU parity: the HIP path (through the C ABI) against the FP64 oracle.
+
+Tolerances (relative L2 against the FP64 oracle):
+  FP64 path: 1e-12 per transform / step, 1e-10 after many steps
+  FP32 path: 2e-6 per transform, 1e-5 after many steps
+Index work (remap, trigger step, halving count) is bit-exact.
+"""
+import numpy as np
+import pytest
+
+from conftest import oracle, rel_l2, random_spectrum, shrt
+
+pytestmark = pytest.mark.gpu
+
+BASE = dict(Lx=2 * np.pi, Ly=2 * np.pi, nu=1e-3, kappa=1e-3, sigma=0.5, g=1.0, rho0=1.0,
+            rho0_prime=1.0, delt=1e-3, cfl_num=0.35, noise=0, rho_eps1=1e-2)
+TOL1 = {8: 1e-12, 4: 2e-6}
+TOLN = {8: 1e-10, 4: 1e-5}
+
+
+def pair(n, prec, ny=None, graph_steps=0, **kw):
+    p = dict(BASE)
+    p.update(kw)
+    ny = ny or n
+    o = oracle.Oracle(nx=n, ny=ny, **p)
+    g = shrt.Shrt(nx=n, ny=ny, precision=prec, graph_steps=graph_steps, **p)
+    return o, g
+
+
+@pytest.mark.parametrize("prec", [8, 4])
+@pytest.mark.parametrize("nx,ny", [(32, 32), (64, 64), (128, 128), (64, 128), (256, 256)])
+def test_xtok_ktox(gpu, prec, nx, ny):
+    print(f"xtok/ktox nx={nx} ny={ny} prec={prec}", flush=True)
+    o, g = pair(nx, prec, ny=ny)
+    rng = np.random.default_rng(nx * 7 + ny)
+    f = rng.standard_normal((nx, ny))
+    assert rel_l2(g.xtok(f), o.xtok(f)) < TOL1[prec]
+    S = random_spectrum(rng, *o.shape_k, decay=0.5, amp=1.0)
+    assert rel_l2(g.ktox(S), o.ktox(S)) < TOL1[prec]
+
+
+@pytest.mark.parametrize("prec", [8, 4])
+def test_xtok_single_mode(gpu, prec):
+    # cos(2x + 10y) has exactly two nonzero retained modes, each rho_eps1/2.
+    n = 64
+    o, g = pair(n, prec)
+    x = np.arange(n) * 2 * np.pi / n
+    f = 1e-2 * np.cos(2 * x[:, None] + 10 * x[None, :])
+    F = g.xtok(f)
+    assert abs(F[2, 10] - 5e-3) < 1e-8
+    assert abs(F[o.s["nkx"] - 2, 10]) < 1e-8
+    F[2, 10] = 0
+    assert np.abs(F).max() < 1e-8
+
+
+@pytest.mark.parametrize("prec", [8, 4])
+@pytest.mark.parametrize("istep", [0, 1, 2, 7])
+def test_one_step_random_state(gpu, prec, istep):
+    """One time_advance from a random sheared state, every scheme branch."""
+    n = 64
+    print(f"one step prec={prec} istep={istep}", flush=True)
+    o, g = pair(n, prec, cfl_num=10.0)
+    rng = np.random.default_rng(100 + istep)
+    nkx, nky = o.shape_k
+    g.set_time(istep, 0.1 * istep, 1e-3)
+    o.set_time(istep, 0.1 * istep, 1e-3)
+    kx, _ = o.k_tables()
+    s = -0.5 * kx * 0.3
+    s[0] = 0.0
+    o.set_shift(s)
+    g.set_shift(s)
+    for ring in ("aomg", "domg", "arho", "drho"):
+        for slot in (0, 1):
+            amp = 1e-2 if ring[0] == "a" else 1e-3
+            arr = random_spectrum(rng, nkx, nky, decay=2.0, amp=amp)
+            if prec == 4:
+                arr = arr.astype(np.complex64).astype(np.complex128)
+            o.set_buffer(ring, slot, arr)
+            g.set_buffer(ring, slot, arr)
+    o.step(1)
+    g.time_advance(1)
+    for ring in ("aomg", "domg", "arho", "drho"):
+        err = rel_l2(g.buffer(ring, 0), o.buffer(ring, 0))
+        print(f"  {ring}0 rel err {err:.3e}")
+        assert err < TOL1[prec] * 5, ring
+    assert rel_l2(g.spectral(shrt.PHI), o.buffer("aphi")) < TOL1[prec] * 5
+    so, sg = o.state(), g.state()
+    assert sg["istep"] == so["istep"] == istep + 1
+    assert sg["delt"] == so["delt"]
+    sh_o, _ = o.shift()
+    sh_g, _ = g.shift()
+    assert np.array_equal(sh_o, sh_g)
+
+
+@pytest.mark.parametrize("prec", [8, 4])
+@pytest.mark.parametrize("n,steps", [(64, 200), (128, 100)])
+def test_multistep_seeded(gpu, prec, n, steps):
+    print(f"multistep n={n} prec={prec} steps={steps}", flush=True)
+    o, g = pair(n, prec, sigma=0.5)
+    ic = dict(seed=12345, kmin=2.0, kmax=8.0, amp=1e-3, expo=-1.0)
+    o.set_spectral("rho", oracle.seeded_spectrum(n, n, 2 * np.pi, 2 * np.pi, **ic))
+    g.initialize_seeded(**ic)
+    r0 = g.spectral(shrt.RHO)
+    assert rel_l2(r0, o.buffer("arho")) < (1e-16 if prec == 8 else 1e-7)
+    o.step(steps)
+    g.time_advance(steps)
+    for f, ring in ((shrt.OMEGA, "aomg"), (shrt.RHO, "arho"), (shrt.PHI, "aphi")):
+        err = rel_l2(g.spectral(f), o.buffer(ring))
+        print(f"  field {f} rel err {err:.3e}")
+        assert err < TOLN[prec]
+    so, sg = o.state(), g.state()
+    assert sg["istep"] == so["istep"]
+    assert abs(sg["time"] - so["time"]) < 1e-12
+
+
+@pytest.mark.parametrize("prec", [8, 4])
+def test_reference_ic_and_steps(gpu, prec):
+    n = 64
+    o, g = pair(n, prec, noise=1, Lx=np.pi, Ly=np.pi, sigma=1.0, kappa=8e-5, cfl_num=0.5)
+    o.initialize_reference()
+    g.initialize()
+    assert rel_l2(g.spectral(shrt.RHO), o.buffer("arho")) < TOL1[prec]
+    o.step(30)
+    g.time_advance(30)
+    for f, ring in ((shrt.OMEGA, "aomg"), (shrt.RHO, "arho")):
+        assert rel_l2(g.spectral(f), o.buffer(ring)) < TOLN[prec]
+
+
+@pytest.mark.parametrize("prec", [8, 4])
+def test_remap_trigger(gpu, prec):
+    """sigma*delt = 0.01 with kx[1] = 1: the remap fires on step index 99."""
+    n = 32
+    print(f"remap trigger prec={prec}", flush=True)
+    o, g = pair(n, prec, sigma=1.0, delt=0.01, cfl_num=5.0, nu=1e-2, kappa=1e-2)
+    ic = dict(seed=7, kmin=1.0, kmax=6.0, amp=1e-3, expo=-1.0)
+    o.set_spectral("rho", oracle.seeded_spectrum(n, n, 2 * np.pi, 2 * np.pi, **ic))
+    g.initialize_seeded(**ic)
+    o.step(100)
+    g.time_advance(100)
+    so, sg = o.state(), g.state()
+    assert so["remap_count"] == 1 and so["last_remap_step"] == 99
+    assert sg["remap_count"] == 1 and sg["last_remap_step"] == 99
+    sh_o, j_o = o.shift()
+    sh_g, j_g = g.shift()
+    assert np.array_equal(sh_o, sh_g)
+    assert np.array_equal(j_o, j_g)
+    # right after the remap every live buffer is read through the remap view
+    for ring in ("aomg", "arho", "domg", "drho"):
+        for slot in (0, 1):
+            err = rel_l2(g.buffer(ring, slot), o.buffer(ring, slot))
+            assert err < TOLN[prec], (ring, slot, err)
+    assert rel_l2(g.spectral(shrt.PHI), o.buffer("aphi")) < TOLN[prec]
+    o.step(20)
+    g.time_advance(20)
+    for f, ring in ((shrt.OMEGA, "aomg"), (shrt.RHO, "arho")):
+        assert rel_l2(g.spectral(f), o.buffer(ring)) < TOLN[prec]
+
+
+@pytest.mark.parametrize("prec", [8, 4])
+def test_shearing_field_bitexact(gpu, prec):
+    n = 64
+    o, g = pair(n, prec)
+    nkx, nky = o.shape_k
+    nkxh = o.s["nkxh"]
+    rng = np.random.default_rng(3)
+    dt = np.complex128 if prec == 8 else np.complex64
+    F = (rng.standard_normal((nkx, nky)) + 1j * rng.standard_normal((nkx, nky))).astype(dt)
+    # the jump table of a remap at sigma*t = 1, and a random one
+    ikx = np.arange(nkx)
+    kxi = np.where(ikx < nkxh, ikx, ikx - nkx)
+    tables = [-kxi, rng.integers(-nky, nky, size=nkx)]
+    tables[1][0] = 0
+    for jmp in tables:
+        jmp = jmp.astype(np.int32)
+        ref = o.shearing_field(jmp, F.astype(np.complex128)).astype(dt)
+        out = g.shearing_field(jmp, F)
+        assert np.array_equal(out.view(np.uint8), ref.view(np.uint8))
+
+
+@pytest.mark.parametrize("prec", [8, 4])
+def test_cfl_halving(gpu, prec):
+    """g/rho0 and velocity checks halve delt on the device exactly like check_cfl."""
+    n = 32
+    o, g = pair(n, prec, delt=0.1, cfl_num=0.2)
+    ic = dict(seed=11, kmin=1.0, kmax=5.0, amp=1e-2, expo=-1.0)
+    o.set_spectral("rho", oracle.seeded_spectrum(n, n, 2 * np.pi, 2 * np.pi, **ic))
+    g.initialize_seeded(**ic)
+    o.step(5)
+    g.time_advance(5)
+    so, sg = o.state(), g.state()
+    assert so["halvings"] > 0
+    assert sg["halvings"] == so["halvings"]
+    assert sg["delt"] == so["delt"]
+    assert len(g.halvings()) >= 1
+    assert rel_l2(g.spectral(shrt.RHO), o.buffer("arho")) < TOLN[prec]
+
+
+@pytest.mark.parametrize("prec", [8, 4])
+def test_linear_mode(gpu, prec):
+    n = 64
+    o, g = pair(n, prec, linear=1)
+    ic = dict(seed=5, kmin=2.0, kmax=10.0, amp=1e-3, expo=-1.0)
+    o.set_spectral("rho", oracle.seeded_spectrum(n, n, 2 * np.pi, 2 * np.pi, **ic))
+    g.initialize_seeded(**ic)
+    o.step(50)
+    g.time_advance(50)
+    for f, ring in ((shrt.OMEGA, "aomg"), (shrt.RHO, "arho")):
+        assert rel_l2(g.spectral(f), o.buffer(ring)) < TOLN[prec]
+
+
+@pytest.mark.parametrize("prec", [8, 4])
+def test_graph_replay_matches_eager(gpu, prec):
+    n = 64
+    ic = dict(seed=99, kmin=2.0, kmax=8.0, amp=1e-3, expo=-1.0)
+    outs = []
+    for gs in (0, 4):
+        g = shrt.Shrt(nx=n, ny=n, precision=prec, graph_steps=gs, **BASE)
+        g.initialize_seeded(**ic)
+        g.time_advance(9)
+        g.time_advance(13)
+        outs.append((g.spectral(shrt.OMEGA), g.spectral(shrt.RHO), g.spectral(shrt.PHI), g.state()))
+        g.close()
+    for a, b in zip(outs[0][:3], outs[1][:3]):
+        assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
+    assert outs[0][3] == outs[1][3]
+
+
+@pytest.mark.parametrize("prec", [8, 4])
+def test_diagnostics_and_physical(gpu, prec):
+    n = 64
+    o, g = pair(n, prec)
+    ic = dict(seed=21, kmin=2.0, kmax=8.0, amp=1e-3, expo=-1.0)
+    o.set_spectral("rho", oracle.seeded_spectrum(n, n, 2 * np.pi, 2 * np.pi, **ic))
+    g.initialize_seeded(**ic)
+    o.step(40)
+    g.time_advance(40)
+    do, dg = o.diagnostics(), g.diagnostics()
+    for k in ("energy", "enstrophy", "rho_var"):
+        assert abs(dg[k] - do[k]) <= TOLN[prec] * abs(do[k]), k
+    for pf, ring in ((shrt.P_OMEGA, "aomg"), (shrt.P_RHO, "arho"), (shrt.P_PHI, "aphi")):
+        assert rel_l2(g.physical(pf), o.ktox(o.buffer(ring))) < TOLN[prec]
+    assert rel_l2(g.physical(shrt.P_VX), o.physical("vx")) < TOLN[prec]
+    assert rel_l2(g.physical(shrt.P_VY), o.physical("vy")) < TOLN[prec]
