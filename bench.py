@@ -1,0 +1,180 @@
+#!/usr/bin/env python3
+"""Throughput of the pseudospectral step on one MI355X.
+
+Workload (the headline "grid-pt*steps/s at 2048^2 FP32"): one instance on a
+2048x2048 grid, FP32 transforms and updates with FP64 shift/time bookkeeping,
+Lx = Ly = 2*pi, sigma = 1, nu = kappa = 1e-4, g = rho0 = rho0' = 1,
+delt = 1e-4, cfl number 0.35; omega = 0 and a seeded random-phase rho on
+2 <= |k| <= 32 with amplitude 1e-4 |k|^-1.5.  Synthetic data; the state lives
+in HBM before the timed region starts.
+
+A step is one time_advance: three fused kernels, replayed from a captured HIP
+graph.  The timed region is `--steps` steps between HIP events on the
+library's stream, after `--warmup` untimed steps.  Per-kernel durations come
+from a second, kernel-by-kernel pass with HIP events between launches; the
+roofline object is for the kernel with the largest average duration.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+PEAK_HBM_GBS = 8000.0
+METRIC = "grid-pt*steps/s at 2048^2 FP32 + HBM roofline %; time-to-error learned vs plain"
+
+
+def _load(name, path):
+    spec = importlib.util.spec_from_file_location(name, path)
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def kernel_bytes(s, c):
+    """Algorithmic HBM bytes per launch of the three step kernels.
+
+    P1 reads omega^n, rho^n (2M) and writes six y-transformed fields (6I);
+    P2 reads those (6I) and writes two bracket spectra (2I); P3 reads them (2I)
+    plus omega^n, rho^n, omega^{n-1}, rho^{n-1} and four tendency histories (8M)
+    and writes omega^{n+1}, rho^{n+1} and two tendencies (4M).  c = bytes per
+    complex element.
+    """
+    M, I = s["M"], s["I"]
+    return {"p1": c * (2 * M + 6 * I), "p2": c * 8 * I, "p3": c * (2 * I + 12 * M)}
+
+
+def read_traffic(name):
+    """HBM bytes per launch measured with rocprofv3 --pmc, if recorded."""
+    path = os.path.join(REPO, "profiles", "traffic.json.txt")
+    if not os.path.exists(path):
+        return None
+    try:
+        d = json.load(open(path))
+        return d.get(name)
+    except Exception:
+        return None
+
+
+def cpu_baseline(orc, n, params, ic, budget_s):
+    """Time the FP64 oracle (the CPU restatement) on a bounded sample."""
+    threads = max(1, min(16, os.cpu_count() or 1))
+    o = orc.Oracle(threads=threads, nx=n, ny=n, **params)
+    o.set_spectral("rho", orc.seeded_spectrum(n, n, params["Lx"], params["Ly"], **ic))
+    t0 = time.perf_counter()
+    o.step(1)
+    one = time.perf_counter() - t0
+    steps = int(max(2, min(50, budget_s / max(one, 1e-6))))
+    t0 = time.perf_counter()
+    o.step(steps)
+    dt = time.perf_counter() - t0
+    o.close()
+    return {"value": n * n * steps / dt, "unit": "grid-pt*steps/s", "cores": threads, "kind": "port",
+            "sample": f"{n}^2 FP64 oracle, seeded IC, {steps} steps after 1 warm step, "
+                      f"{threads} OpenMP threads, {dt:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--n", type=int, default=2048)
+    ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
+    ap.add_argument("--graph-steps", type=int, default=20)
+    ap.add_argument("--kernel-steps", type=int, default=50)
+    ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU-baseline work")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} (WORLD_SIZE {world}) is not supported; "
+              "the single-instance step is not sharded across GPUs", file=sys.stderr)
+        sys.exit(2)
+
+    shrt = _load("shrt_binding", os.path.join(REPO, "shearing-rt_amd", "shrt.py"))
+    orc = _load("shrt_oracle_py", os.path.join(REPO, "oracle", "oracle.py"))
+    if shrt.lib().shrt_device_count() < 1:
+        raise RuntimeError("no HIP device visible")
+
+    n = args.n
+    prec = shrt.FP32 if args.precision == "fp32" else shrt.FP64
+    params = dict(Lx=2 * np.pi, Ly=2 * np.pi, delt=1e-4, cfl_num=0.35, nu=1e-4, kappa=1e-4,
+                  sigma=1.0, g=1.0, rho0=1.0, rho0_prime=1.0, rho_eps1=1e-2, noise=0, linear=0)
+    ic = dict(seed=20250101, kmin=2.0, kmax=32.0, amp=1e-4, expo=-1.5)
+
+    log(f"bench: {n}^2 {args.precision}, graph_steps={args.graph_steps}, warmup={args.warmup}, steps={args.steps}")
+    g = shrt.Shrt(nx=n, ny=n, precision=prec, graph_steps=args.graph_steps, **params)
+    g.initialize_seeded(**ic)
+    if args.warmup > 0:
+        g.time_advance(args.warmup)
+    g.synchronize()
+    t0 = time.perf_counter()
+    ms = g.time_steps(args.steps)
+    wall = time.perf_counter() - t0
+    st = g.state()
+    log(f"bench: timed {args.steps} steps: {ms:.3f} ms (events), {wall * 1e3:.3f} ms (wall); state {st}")
+    kt = g.kernel_times(args.kernel_steps)
+    log(f"bench: kernel ms per launch p1={kt[0]:.4f} p2={kt[1]:.4f} p3={kt[2]:.4f}")
+    g.close()
+
+    c = 8 if prec == shrt.FP32 else 16
+    s = shrt.sizes(n, n)
+    kb = kernel_bytes(s, c)
+    names = ["p1", "p2", "p3"]
+    dom = int(np.argmax(kt))
+    dname = names[dom]
+    achieved = kb[dname] / (kt[dom] * 1e-3) / 1e9
+    traffic = read_traffic(f"{dname}_{n}_{args.precision}")
+    ms_step = ms / args.steps
+    step_bytes = sum(kb.values())
+    baseline_formula = c * (14 * s["M"] + 10 * s["I"])
+
+    out = {
+        "metric": METRIC,
+        "value": n * n * args.steps / (ms * 1e-3),
+        "unit": "grid-pt*steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.precision,
+        "data": "synthetic: omega=0, seeded random-phase rho on 2<=|k|<=32, amplitude 1e-4|k|^-1.5",
+        "config": {"workload": f"pseudospectral step, {n}^2 {args.precision.upper()} single instance "
+                               "(sigma=1, nu=kappa=1e-4, g=1, delt=1e-4)",
+                   "nx": n, "ny": n, "graph_steps": args.graph_steps},
+        "roofline": {"bound": "hbm", "kernel": dname, "achieved": achieved, "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
+                     "bytes_per_launch": kb[dname], "ms_per_launch": float(kt[dom])},
+        "kernels_ms": {k: float(v) for k, v in zip(names, kt)},
+        "step_roofline": {"bytes_per_step": step_bytes,
+                          "frac": step_bytes / (ms_step * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                          "bytes_per_step_c14M10I": baseline_formula,
+                          "frac_c14M10I": baseline_formula / (ms_step * 1e-3) / 1e9 / PEAK_HBM_GBS},
+        "final_state": {"istep": st["istep"], "time": st["time"], "delt": st["delt"],
+                        "remaps": st["remap_count"], "halvings": st["halvings"]},
+    }
+    if not args.no_cpu_baseline:
+        log("bench: CPU baseline (FP64 oracle) ...")
+        out["cpu_baseline"] = cpu_baseline(orc, n, params, ic, args.cpu_budget)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

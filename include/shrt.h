@@ -159,6 +159,16 @@ shrt_status shrt_diagnostics(shrt_ctx* ctx, shrt_diag* out);
  * time_integral.cu:363-382).  Returns the number copied (<= cap) in *count. */
 shrt_status shrt_get_halvings(shrt_ctx* ctx, shrt_halving* out, int cap, int* count);
 
+/* Benchmark support (no reference counterpart).  shrt_time_steps enqueues
+ * `nsteps` steps exactly as shrt_step_async does (graph replays when
+ * graph_steps > 0) between two HIP events on the context's stream and returns
+ * the elapsed milliseconds.  shrt_kernel_times launches `nsteps` steps kernel
+ * by kernel with HIP events between the launches and returns the average
+ * duration in milliseconds of each of the three step kernels (ms[0..2] =
+ * y-inverse/Poisson pass, x pass with brackets, y-forward/update pass). */
+shrt_status shrt_time_steps(shrt_ctx* ctx, int nsteps, double* ms_total);
+shrt_status shrt_kernel_times(shrt_ctx* ctx, int nsteps, double* ms3);
+
 /* Sizes derived by init_fft (fft.cu:91-96): out[0..7] = nkx, nky, nkxh, nkxh2,
  * nkxpad, ncy, nkx*nky, nkxh*ny. */
 shrt_status shrt_sizes(int nx, int ny, int* out8);

@@ -557,6 +557,70 @@ shrt_status shrt_step(shrt_ctx* c, int nsteps) {
     return shrt_synchronize(c);
 }
 
+shrt_status shrt_time_steps(shrt_ctx* c, int nsteps, double* ms_total) {
+    if (!c || !ms_total || nsteps < 0) return fail(SHRT_ERR_INVALID, "bad arguments");
+    // build the graphs outside the timed region
+    if (c->p.graph_steps >= 2 && nsteps >= c->p.graph_steps) {
+        shrt_status st = build_graph(c, c->cur);
+        if (st != SHRT_OK) return st;
+    }
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, c->stream));
+    shrt_status st = shrt_step_async(c, nsteps);
+    HIP_TRY(hipEventRecord(e1, c->stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st != SHRT_OK) return st;
+    *ms_total = ms;
+    return shrt_synchronize(c);
+}
+
+shrt_status shrt_kernel_times(shrt_ctx* c, int nsteps, double* ms3) {
+    if (!c || !ms3 || nsteps <= 0) return fail(SHRT_ERR_INVALID, "bad arguments");
+    std::vector<hipEvent_t> ev((size_t)nsteps * 4);
+    for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+    hipError_t err = hipSuccess;
+    for (int k = 0; k < nsteps && err == hipSuccess; ++k) {
+        hipEvent_t* e = &ev[(size_t)k * 4];
+        const int par = c->cur;
+        auto launch = [&](int which) -> hipError_t {
+            if (c->prec == 4) {
+                StepArgs<float> a = step_args<float>(c, par, k == nsteps - 1);
+                return which == 0 ? Launch<float>::p1(a, c->g.ntile_x * (c->ph.linear ? 1 : 3), c->stream)
+                     : which == 1 ? Launch<float>::p2(a, c->g.ntile_y, c->stream)
+                                  : Launch<float>::p3(a, c->g.ntile_x, c->stream);
+            }
+            StepArgs<double> a = step_args<double>(c, par, k == nsteps - 1);
+            return which == 0 ? Launch<double>::p1(a, c->g.ntile_x * (c->ph.linear ? 1 : 3), c->stream)
+                 : which == 1 ? Launch<double>::p2(a, c->g.ntile_y, c->stream)
+                              : Launch<double>::p3(a, c->g.ntile_x, c->stream);
+        };
+        for (int w = 0; w < 3 && err == hipSuccess; ++w) {
+            err = hipEventRecord(e[w], c->stream);
+            if (err == hipSuccess) err = launch(w);
+        }
+        if (err == hipSuccess) err = hipEventRecord(e[3], c->stream);
+        c->cur ^= 1;
+    }
+    HIP_TRY(err);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    double acc[3] = {0, 0, 0};
+    for (int k = 0; k < nsteps; ++k)
+        for (int w = 0; w < 3; ++w) {
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, ev[(size_t)k * 4 + w], ev[(size_t)k * 4 + w + 1]));
+            acc[w] += ms;
+        }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    for (int w = 0; w < 3; ++w) ms3[w] = acc[w] / nsteps;
+    return shrt_synchronize(c);
+}
+
 shrt_status shrt_get_time(shrt_ctx* c, shrt_time* out) {
     if (!c || !out) return fail(SHRT_ERR_INVALID, "null argument");
     StepCtl v;

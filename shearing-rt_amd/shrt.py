@@ -89,6 +89,8 @@ _SIGS = {
     "shrt_diagnostics": ([_VP, ctypes.POINTER(Diag)], ctypes.c_int),
     "shrt_get_halvings": ([_VP, ctypes.POINTER(Halving), ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
                           ctypes.c_int),
+    "shrt_time_steps": ([_VP, ctypes.c_int, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    "shrt_kernel_times": ([_VP, ctypes.c_int, _VP], ctypes.c_int),
     "shrt_sizes": ([ctypes.c_int, ctypes.c_int, _VP], ctypes.c_int),
     "shrt_supported": ([ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "shrt_device_count": ([], ctypes.c_int),
@@ -252,6 +254,18 @@ class Shrt:
         d = Diag()
         _check(lib().shrt_diagnostics(self.h, ctypes.byref(d)))
         return dict(energy=d.energy, enstrophy=d.enstrophy, rho_var=d.rho_var)
+
+    def time_steps(self, n):
+        """Milliseconds (HIP events on the context stream) for n steps."""
+        ms = ctypes.c_double()
+        _check(lib().shrt_time_steps(self.h, int(n), ctypes.byref(ms)))
+        return ms.value
+
+    def kernel_times(self, n):
+        """Average milliseconds per launch of the three step kernels over n steps."""
+        out = np.zeros(3)
+        _check(lib().shrt_kernel_times(self.h, int(n), _ptr(out)))
+        return out
 
     def halvings(self, cap=256):
         arr = (Halving * cap)()
