@@ -17,6 +17,7 @@ roofline object is for the kernel with the largest average duration.
 Prints one JSON line on stdout; progress goes to stderr.
 """
 import argparse
+import ctypes
 import importlib.util
 import json
 import os
@@ -95,6 +96,8 @@ def main():
     ap.add_argument("--kernel-steps", type=int, default=50)
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU-baseline work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibrate", type=int, default=0, metavar="BYTES",
+                    help="only run the plain copy kernel on BYTES (the HBM-counter reference) and exit")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -107,6 +110,18 @@ def main():
     orc = _load("shrt_oracle_py", os.path.join(REPO, "oracle", "oracle.py"))
     if shrt.lib().shrt_device_count() < 1:
         raise RuntimeError("no HIP device visible")
+
+    def copy_gbs(nbytes, reps):
+        ms = ctypes.c_double()
+        st = shrt.lib().shrt_copy_bench(nbytes, reps, ctypes.byref(ms))
+        if st != 0:
+            raise RuntimeError(shrt.lib().shrt_last_error().decode())
+        return ms.value, 2 * nbytes / (ms.value * 1e-3) / 1e9
+
+    if args.copy_calibrate:
+        ms, gbs = copy_gbs(args.copy_calibrate, 5)
+        print(json.dumps({"copy_bytes": args.copy_calibrate, "ms": ms, "GB/s": gbs}), flush=True)
+        return
 
     n = args.n
     prec = shrt.FP32 if args.precision == "fp32" else shrt.FP64
@@ -128,6 +143,8 @@ def main():
     kt = g.kernel_times(args.kernel_steps)
     log(f"bench: kernel ms per launch p1={kt[0]:.4f} p2={kt[1]:.4f} p3={kt[2]:.4f}")
     g.close()
+    copy_ms, copy_bw = copy_gbs(1 << 30, 10)
+    log(f"bench: stream copy 1 GiB: {copy_ms:.3f} ms, {copy_bw:.1f} GB/s (read + write)")
 
     c = 8 if prec == shrt.FP32 else 16
     s = shrt.sizes(n, n)
@@ -161,6 +178,7 @@ def main():
                      "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
                      "bytes_per_launch": kb[dname], "ms_per_launch": float(kt[dom])},
         "kernels_ms": {k: float(v) for k, v in zip(names, kt)},
+        "stream_copy_GBs": copy_bw,
         "step_roofline": {"bytes_per_step": step_bytes,
                           "frac": step_bytes / (ms_step * 1e-3) / 1e9 / PEAK_HBM_GBS,
                           "bytes_per_step_c14M10I": baseline_formula,

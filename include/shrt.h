@@ -169,6 +169,12 @@ shrt_status shrt_get_halvings(shrt_ctx* ctx, shrt_halving* out, int cap, int* co
 shrt_status shrt_time_steps(shrt_ctx* ctx, int nsteps, double* ms_total);
 shrt_status shrt_kernel_times(shrt_ctx* ctx, int nsteps, double* ms3);
 
+/* Benchmark support (no reference counterpart): copy `bytes` (a multiple of 16)
+ * between two fresh device buffers with a plain copy kernel, `reps` times, and
+ * return the average milliseconds per copy.  Its HBM counters are the
+ * reference point that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings. */
+shrt_status shrt_copy_bench(long long bytes, int reps, double* ms);
+
 /* Sizes derived by init_fft (fft.cu:91-96): out[0..7] = nkx, nky, nkxh, nkxh2,
  * nkxpad, ncy, nkx*nky, nkxh*ny. */
 shrt_status shrt_sizes(int nx, int ny, int* out8);

@@ -88,10 +88,7 @@ k_aux_yfwd(AuxArgs<T> a) {
     __syncthreads();
     cplx<T> v[P];
 #pragma unroll
-    for (int m = 0; m < P; ++m) {
-        const int e = tid + m * TT;
-        v[m] = sm[(e >> 2) * 16 + team * 4 + (e & 3)];
-    }
+    for (int m = 0; m < P; ++m) v[m] = sm[stage_index(tid, team) + 4 * TT * m];
     __syncthreads();
     TeamFFT<T, NY, P, S::RMAX, -1>::run(v, lds, a.tw, tid, sync);
 #pragma unroll
@@ -159,10 +156,7 @@ k_aux_yinv(AuxArgs<T> a) {
     }
     TeamFFT<T, NY, P, S::RMAX, +1>::run(v, lds, a.tw, tid, sync);
 #pragma unroll
-    for (int m = 0; m < P; ++m) {
-        const int e = tid + m * TT;
-        sm[(e >> 2) * 16 + team * 4 + (e & 3)] = v[m];
-    }
+    for (int m = 0; m < P; ++m) sm[stage_index(tid, team) + 4 * TT * m] = v[m];
     __syncthreads();
     cplx<T>* Gf = static_cast<cplx<T>*>(a.out);
     for (int idx = threadIdx.x; idx < NY * 4; idx += blockDim.x) {
@@ -227,6 +221,14 @@ __global__ void k_phi_from_omega(Geo g, const cplx<T>* __restrict__ om, cplx<T>*
     const T k2 = (T)k2_of(kx[row], kys_of(ky, s, row, iky));
     const cplx<T> w = om[t];
     phi[t] = {w.x / k2, w.y / k2};
+}
+
+// Plain 16-byte copy: the reference point for HBM traffic counters
+// (reads and writes exactly `n` 16-byte words).
+template <typename T>
+__global__ void k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, long long n) {
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
+        dst[t] = src[t];
 }
 
 template <typename T>

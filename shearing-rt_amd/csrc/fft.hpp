@@ -159,11 +159,32 @@ struct TeamFFT {
 #pragma unroll
             for (int r = 0; r < R; ++r) u[r] = v[q + r * Q];
             if constexpr (NS > 1) {
+                // w_r = W_{NS*R}^{r*k}: w_1 (and w_4 for R >= 8) from the table,
+                // the others as short products of those (relative error <= ~8 ulp).
+                // Each power is applied as soon as it is formed; only w_1..w_4
+                // and the multiples of 4 stay live.
+                constexpr int UNIT = N / (NS * R);
+                cplx<T> w1 = tw[k * UNIT];
+                if constexpr (D > 0) w1.y = -w1.y;
+                u[1] = cmul(u[1], w1);
+                if constexpr (R >= 4) {
+                    const cplx<T> w2 = cmul(w1, w1);
+                    const cplx<T> w3 = cmul(w2, w1);
+                    u[2] = cmul(u[2], w2);
+                    u[3] = cmul(u[3], w3);
+                    if constexpr (R >= 8) {
+                        cplx<T> w4 = tw[4 * k * UNIT];
+                        if constexpr (D > 0) w4.y = -w4.y;
+                        cplx<T> w4m = w4;   // w_{4j}
 #pragma unroll
-                for (int r = 1; r < R; ++r) {
-                    cplx<T> w = tw[r * k * (N / (NS * R))];
-                    if constexpr (D > 0) w.y = -w.y;
-                    u[r] = cmul(u[r], w);
+                        for (int j = 1; j < R / 4; ++j) {
+                            if (j > 1) w4m = cmul(w4m, w4);
+                            u[4 * j] = cmul(u[4 * j], w4m);
+                            u[4 * j + 1] = cmul(u[4 * j + 1], cmul(w4m, w1));
+                            u[4 * j + 2] = cmul(u[4 * j + 2], cmul(w4m, w2));
+                            u[4 * j + 3] = cmul(u[4 * j + 3], cmul(w4m, w3));
+                        }
+                    }
                 }
             }
             dft<R, D>(u);
@@ -171,15 +192,26 @@ struct TeamFFT {
 #pragma unroll
                 for (int r = 0; r < R; ++r) v[q + r * Q] = u[r];
             } else {
-                const int base = (b / NS) * NS * R + k;
+                // lpad(B0 + k + r*NS) = [B0 + k + (B0 >> 5) + (k >> 5)] + [r*NS + (r*NS >> 5)]
+                // for the block base B0 = (b/NS)*NS*R (a multiple of 32, or a block
+                // inside one 32-element segment) and k < NS: the second bracket is a
+                // compile-time constant, so the stores use immediate offsets.
+                const int B0 = (b / NS) * NS * R;
+                cplx<T>* dst = lds + (B0 + k + (B0 >> 5) + (k >> 5));
 #pragma unroll
-                for (int r = 0; r < R; ++r) lds[lpad(base + r * NS)] = u[r];
+                for (int r = 0; r < R; ++r) dst[r * NS + ((r * NS) >> 5)] = u[r];
             }
         }
         if constexpr (!LAST) {
             sync();
+            if constexpr (TT % 32 == 0) {
+                const cplx<T>* src = lds + (tid + (tid >> 5));
 #pragma unroll
-            for (int m = 0; m < P; ++m) v[m] = lds[lpad(tid + m * TT)];
+                for (int m = 0; m < P; ++m) v[m] = src[m * (TT + TT / 32)];
+            } else {
+#pragma unroll
+                for (int m = 0; m < P; ++m) v[m] = lds[lpad(tid + m * TT)];
+            }
             sync();
             stages<NS * R>(v, lds, tw, tid, sync);
         }
@@ -195,7 +227,9 @@ struct TeamFFT {
 // Team geometry for a transform length: P points per thread, TT threads.
 template <int N>
 struct TeamShape {
-    static constexpr int P = N / 64 > 32 ? 32 : (N / 64 < 8 ? (N < 8 ? N : 8) : N / 64);
+    // 16 points per thread from N = 1024 up (2 wavefronts per team at 2048,
+    // 4 at 4096), 8 below; at most N/4 threads per team.
+    static constexpr int P = N >= 1024 ? 16 : (N < 8 ? N : 8);
     static constexpr int TT = N / P;
     static constexpr int RMAX = 16;
 };

@@ -156,7 +156,7 @@ hipError_t enqueue_step_t(shrt_ctx* c, int in_par, int store_phi) {
     if (e != hipSuccess) return e;
     e = Launch<T>::p2(a, c->g.ntile_y, c->stream);
     if (e != hipSuccess) return e;
-    return Launch<T>::p3(a, c->g.ntile_x, c->stream);
+    return Launch<T>::p3(a, 2 * c->g.ntile_x, c->stream);
 }
 
 hipError_t enqueue_step(shrt_ctx* c, int in_par, int store_phi) {
@@ -580,6 +580,36 @@ shrt_status shrt_time_steps(shrt_ctx* c, int nsteps, double* ms_total) {
     return shrt_synchronize(c);
 }
 
+shrt_status shrt_copy_bench(long long bytes, int reps, double* ms) {
+    if (bytes <= 0 || bytes % 16 != 0 || reps <= 0 || !ms) return fail(SHRT_ERR_INVALID, "bad arguments");
+    void *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) {
+        if (a) (void)hipFree(a);
+        (void)hipStreamDestroy(st);
+        return fail(SHRT_ERR_ALLOC, "copy bench allocation failed");
+    }
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipMemsetAsync(a, 1, bytes, st));
+    HIP_TRY(Launch<float>::copy16(a, b, bytes / 16, st));   // warm
+    HIP_TRY(hipEventRecord(e0, st));
+    for (int i = 0; i < reps; ++i) HIP_TRY(Launch<float>::copy16(a, b, bytes / 16, st));
+    HIP_TRY(hipEventRecord(e1, st));
+    HIP_TRY(hipEventSynchronize(e1));
+    float t = 0.f;
+    HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+    *ms = t / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    (void)hipStreamDestroy(st);
+    return SHRT_OK;
+}
+
 shrt_status shrt_kernel_times(shrt_ctx* c, int nsteps, double* ms3) {
     if (!c || !ms3 || nsteps <= 0) return fail(SHRT_ERR_INVALID, "bad arguments");
     std::vector<hipEvent_t> ev((size_t)nsteps * 4);
@@ -593,12 +623,12 @@ shrt_status shrt_kernel_times(shrt_ctx* c, int nsteps, double* ms3) {
                 StepArgs<float> a = step_args<float>(c, par, k == nsteps - 1);
                 return which == 0 ? Launch<float>::p1(a, c->g.ntile_x * (c->ph.linear ? 1 : 3), c->stream)
                      : which == 1 ? Launch<float>::p2(a, c->g.ntile_y, c->stream)
-                                  : Launch<float>::p3(a, c->g.ntile_x, c->stream);
+                                  : Launch<float>::p3(a, 2 * c->g.ntile_x, c->stream);
             }
             StepArgs<double> a = step_args<double>(c, par, k == nsteps - 1);
             return which == 0 ? Launch<double>::p1(a, c->g.ntile_x * (c->ph.linear ? 1 : 3), c->stream)
                  : which == 1 ? Launch<double>::p2(a, c->g.ntile_y, c->stream)
-                              : Launch<double>::p3(a, c->g.ntile_x, c->stream);
+                              : Launch<double>::p3(a, 2 * c->g.ntile_x, c->stream);
         };
         for (int w = 0; w < 3 && err == hipSuccess; ++w) {
             err = hipEventRecord(e[w], c->stream);

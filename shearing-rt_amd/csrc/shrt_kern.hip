@@ -149,6 +149,13 @@ hipError_t Launch<R>::fill_zero(void* p, long long n, hipStream_t s) {
 }
 
 template <>
+hipError_t Launch<R>::copy16(const void* src, void* dst, long long nwords, hipStream_t s) {
+    hipLaunchKernelGGL((k_copy16<R>), dim3(4096), dim3(256), 0, s, static_cast<const uint4*>(src),
+                       static_cast<uint4*>(dst), nwords);
+    return hipGetLastError();
+}
+
+template <>
 hipError_t Launch<R>::diag(const Geo& g, const void* om, const void* rh, const int* jmp, const double* kx,
                            const double* ky, const double* sh, double* partial, int nblocks, double* out,
                            hipStream_t s) {

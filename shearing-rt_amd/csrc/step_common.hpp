@@ -20,6 +20,18 @@ namespace shrt {
 
 constexpr int kTile = 4;   // rows (kx) per P1/P3 workgroup == columns (y) per P2 workgroup
 
+// Waves per SIMD that the LDS of a kTile-team workgroup allows (160 KiB per
+// CU, four SIMDs), capped at 4 (128 VGPRs).  Used as the occupancy target
+// of the transform kernels so the compiler keeps their registers within it.
+template <typename T, int N>
+constexpr int waves_per_eu() {
+    constexpr long long lds = (long long)kTile * lds_len<N>() * (long long)sizeof(cplx<T>);
+    constexpr long long wgs = (160LL * 1024LL) / lds;
+    constexpr long long waves = wgs * ((kTile * TeamShape<N>::TT + 63) / 64);
+    constexpr long long w = waves / 4;
+    return w < 1 ? 1 : (w > 4 ? 4 : (int)w);
+}
+
 struct Geo {
     int nx, ny, nkx, nky, nkxh, nkxh2, nkxpad, ncy;
     int ntile_x;   // ceil(nkxh / kTile): row-pair tiles
@@ -91,6 +103,11 @@ __host__ __device__ inline int slot_a1(long long n) { return (int)(((-n) % 3 + 3
 __host__ __device__ inline int slot_a2(long long n) { return (int)(((1 - n) % 3 + 3) % 3); }
 
 template <typename T> __device__ __forceinline__ cplx<T> czero() { return {T(0), T(0)}; }
+
+// Position of element e = tid + m*TT of team `team` in a workgroup staging
+// buffer laid out [e/4][team][e%4] (the G/H tile order): the thread part plus
+// 4*TT*m, valid because TT is a multiple of 4.
+__device__ __forceinline__ int stage_index(int tid, int team) { return (tid >> 2) * 16 + team * 4 + (tid & 3); }
 
 // Remapped read of a spectral buffer (shear.cu:296-322).  `jmp` is null when the
 // buffer is already in the current frame.

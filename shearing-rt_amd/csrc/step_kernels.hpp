@@ -28,117 +28,136 @@ namespace shrt {
 // ---------------------------------------------------------------------------
 // P1
 // ---------------------------------------------------------------------------
-// Spectral multiplier of output `o` of group `grp` applied to F at (row, iky).
+// Sheared wavenumbers of one mode in the working precision.  FP64 keeps the
+// reference's operation order (shear.cu:335-336); FP32 rounds ky + s once and
+// forms kperp2 in FP32.
 template <typename T>
-__device__ __forceinline__ cplx<T> p1_mult(int grp, int o, const double* __restrict__ ky,
-                                           const double* __restrict__ s, int row, int iky, double kxv,
-                                           cplx<T> F) {
-    const double kys = kys_of(ky, s, row, iky);
-    if (grp == 0) {
-        cplx<T> phi;
-        if (row == 0 && iky == 0) {
-            phi = czero<T>();
-        } else {
-            const T k2 = (T)k2_of(kxv, kys);
-            phi = {F.x / k2, F.y / k2};                      // neg_lapinv_shear
-        }
-        if (o == 0) { const T k = (T)kys; return {-k * phi.y, k * phi.x}; }   // ddy_shear -> vx
-        const T k = (T)kxv;
-        return {k * phi.y, -k * phi.x};                                        // neg_ddx -> vy
-    }
-    if (o == 0) { const T k = (T)kxv; return {-k * F.y, k * F.x}; }           // ddx
-    const T k = (T)kys;
-    return {-k * F.y, k * F.x};                                                // ddy_shear
+struct ModeK {
+    T kx, kys, k2;
+};
+
+template <typename T>
+__device__ __forceinline__ ModeK<T> mode_k(const double* __restrict__ ky, double s_row, double kxv, int iky) {
+    const double kysd = __dadd_rn(ky[iky], s_row);
+    ModeK<T> m;
+    m.kx = (T)kxv;
+    m.kys = (T)kysd;
+    if constexpr (sizeof(T) == 8) m.k2 = (T)k2_of(kxv, kysd);
+    else m.k2 = m.kx * m.kx + m.kys * m.kys;
+    return m;
 }
 
-template <typename T, int NY>
-__global__ void __launch_bounds__(kTile * TeamShape<NY>::TT)
-k_p1(StepArgs<T> a) {
+// Multiplier of output O of group GRP: group 0 takes phi = omega/kperp2
+// (neg_lapinv_shear, shear.cu:367-374; phi = 0 at the origin) to vx = i kys phi
+// (ddy_shear, shear.cu:342-354) and vy = -i kx phi (neg_ddx, fourier.cu:190-198);
+// groups 1 and 2 take omega or rho to i kx f (ddx, fourier.cu:180-188) and i kys f.
+template <int GRP, int O, typename T>
+__device__ __forceinline__ cplx<T> p1_mult(const ModeK<T>& k, bool origin, cplx<T> F) {
+    if constexpr (GRP == 0) {
+        const cplx<T> phi = origin ? czero<T>() : cplx<T>{F.x / k.k2, F.y / k.k2};
+        if constexpr (O == 0) return {-k.kys * phi.y, k.kys * phi.x};
+        else return {k.kx * phi.y, -k.kx * phi.x};
+    } else {
+        if constexpr (O == 0) return {-k.kx * F.y, k.kx * F.x};
+        else return {-k.kys * F.y, k.kys * F.x};
+    }
+}
+
+// One P1 workgroup: kTile row pairs (kx, -kx) of group GRP, two outputs each.
+template <int GRP, typename T, int NY>
+__device__ __forceinline__ void p1_body(const StepArgs<T>& a, cplx<T>* sm, int team, int tid, int tile) {
     using S = TeamShape<NY>;
     constexpr int P = S::P, TT = S::TT, L = lds_len<NY>();
-    __shared__ cplx<T> sm[kTile * L];
     const Geo& g = a.g;
-    const int team = threadIdx.x / TT, tid = threadIdx.x - team * TT;
-    const int tile = blockIdx.x % g.ntile_x;
-    const int grp = blockIdx.x / g.ntile_x;   // 0: phi -> (vx, vy); 1: omega; 2: rho
     const int row = tile * kTile + team;
     const bool act = row < g.nkxh;
-    const int rn = row == 0 ? 0 : g.nkx - row;
+    const int rowc = act ? row : 0;
+    const int rn = rowc == 0 ? 0 : g.nkx - rowc;
     const StepCtl* c = a.ctl_in;
-    const long long n = c->istep;
-    const int s1 = slot_a1(n);
-    const cplx<T>* src = grp == 2 ? a.a_rho[s1] : a.a_omg[s1];
+    const int s1 = slot_a1(c->istep);
+    const cplx<T>* src = GRP == 2 ? a.a_rho[s1] : a.a_omg[s1];
     const int* jmp = (c->slot_epoch_a[s1] != c->epoch) ? a.jump + (c->epoch & 1) * g.nkx : nullptr;
     cplx<T>* lds = sm + team * L;
     auto sync = [] { __syncthreads(); };
-    const double kxp = act ? a.kx[row] : 0.0;
-    const double kxn = act ? a.kx[rn] : 0.0;
+    const double kxp = a.kx[rowc], kxn = a.kx[rn];
+    const double sp = a.s_in[rowc], sn = a.s_in[rn];
     const int nrow_tile = min(kTile, g.nkxh - tile * kTile);
 
-#pragma unroll 1
-    for (int o = 0; o < 2; ++o) {
+    static_for<0, 2>([&](auto oc) {
+        constexpr int O = decltype(oc)::value;
+        // y-spectrum of the row pair in natural order, branch-free: element e
+        // takes row kx at iky = e (e < nky) or the conjugate of row -kx at
+        // iky = NY - e (e > NY - nky); padding lanes load an in-bounds element
+        // and are zeroed.
         cplx<T> v[P];
 #pragma unroll
         for (int m = 0; m < P; ++m) {
             const int e = tid + m * TT;
-            cplx<T> val = czero<T>();
-            if (act) {
-                if (e < g.nky) {
-                    val = p1_mult<T>(grp, o, a.ky, a.s_in, row, e, kxp, view_load(src, row, e, g, jmp));
-                    if (e == 0) {
-                        // Hermitian part of the ky=0 column, as the x-first 2-D C2R uses it.
-                        const cplx<T> vn = p1_mult<T>(grp, o, a.ky, a.s_in, rn, 0, kxn, view_load(src, rn, 0, g, jmp));
-                        val = {(val.x + vn.x) * T(0.5), (val.y - vn.y) * T(0.5)};
-                    }
-                } else if (e > NY - g.nky) {
-                    const int iky = NY - e;
-                    const cplx<T> vn = p1_mult<T>(grp, o, a.ky, a.s_in, rn, iky, kxn, view_load(src, rn, iky, g, jmp));
-                    val = {vn.x, -vn.y};
-                }
-            }
-            v[m] = val;
+            const bool pos = e < g.nky;
+            const bool keep = act && (pos || e > NY - g.nky);
+            const int iky = keep ? (pos ? e : NY - e) : 0;
+            const int r = pos ? rowc : rn;
+            const cplx<T> F = view_load(src, r, iky, g, jmp);
+            const ModeK<T> k = mode_k<T>(a.ky, pos ? sp : sn, pos ? kxp : kxn, iky);
+            const cplx<T> z = p1_mult<GRP, O, T>(k, r == 0 && iky == 0, F);
+            v[m] = keep ? (pos ? z : cplx<T>{z.x, -z.y}) : czero<T>();
+        }
+        if (act && tid == 0) {
+            // Hermitian part of the ky=0 column, as the x-first 2-D C2R uses it.
+            const ModeK<T> k = mode_k<T>(a.ky, sn, kxn, 0);
+            const cplx<T> vn = p1_mult<GRP, O, T>(k, rn == 0, view_load(src, rn, 0, g, jmp));
+            v[0] = {(v[0].x + vn.x) * T(0.5), (v[0].y - vn.y) * T(0.5)};
         }
         TeamFFT<T, NY, P, S::RMAX, +1>::run(v, lds, a.twy, tid, sync);
         // stage as [t][team][c] so each 4x4 block is one contiguous run of G
+        const int st = stage_index(tid, team);
 #pragma unroll
-        for (int m = 0; m < P; ++m) {
-            const int e = tid + m * TT;
-            sm[(e >> 2) * 16 + team * 4 + (e & 3)] = v[m];
-        }
+        for (int m = 0; m < P; ++m) sm[st + 4 * TT * m] = v[m];
         __syncthreads();
-        cplx<T>* Gf = a.G + (long long)(grp * 2 + o) * g.I;
+        cplx<T>* Gf = a.G + (long long)(GRP * 2 + O) * g.I;
         for (int idx = threadIdx.x; idx < NY * 4; idx += blockDim.x) {
             const int t = idx >> 4, w = idx & 15;
             if ((w >> 2) < nrow_tile) Gf[((long long)t * g.nkxh + tile * kTile) * 4 + w] = sm[idx];
         }
         __syncthreads();
-    }
+    });
 
     // aphi (shear.cu:367-374) for the rows of this tile, when requested
-    if (grp == 0 && a.store_phi && act) {
-        for (int iky = tid; iky < g.nky; iky += TT) {
+    if constexpr (GRP == 0) {
+        if (a.store_phi && act) {
+            for (int iky = tid; iky < g.nky; iky += TT) {
 #pragma unroll
-            for (int side = 0; side < 2; ++side) {
-                if (side == 1 && row == 0) break;
-                const int r = side ? rn : row;
-                const double kxv = side ? kxn : kxp;
-                cplx<T> phi = czero<T>();
-                if (!(r == 0 && iky == 0)) {
+                for (int side = 0; side < 2; ++side) {
+                    if (side == 1 && row == 0) break;
+                    const int r = side ? rn : row;
+                    const ModeK<T> k = mode_k<T>(a.ky, side ? sn : sp, side ? kxn : kxp, iky);
                     const cplx<T> F = view_load(src, r, iky, g, jmp);
-                    const T k2 = (T)k2_of(kxv, kys_of(a.ky, a.s_in, r, iky));
-                    phi = {F.x / k2, F.y / k2};
+                    a.aphi[(long long)r * g.nky + iky] =
+                        (r == 0 && iky == 0) ? czero<T>() : cplx<T>{F.x / k.k2, F.y / k.k2};
                 }
-                a.aphi[(long long)r * g.nky + iky] = phi;
             }
         }
     }
+}
+
+template <typename T, int NY>
+__global__ void __launch_bounds__(kTile * TeamShape<NY>::TT, (waves_per_eu<T, NY>()))
+k_p1(StepArgs<T> a) {
+    using S = TeamShape<NY>;
+    __shared__ cplx<T> sm[kTile * lds_len<NY>()];
+    const int team = threadIdx.x / S::TT, tid = threadIdx.x - team * S::TT;
+    const int tile = blockIdx.x % a.g.ntile_x;
+    const int grp = blockIdx.x / a.g.ntile_x;   // 0: phi -> (vx, vy); 1: omega; 2: rho
+    if (grp == 0) p1_body<0, T, NY>(a, sm, team, tid, tile);
+    else if (grp == 1) p1_body<1, T, NY>(a, sm, team, tid, tile);
+    else p1_body<2, T, NY>(a, sm, team, tid, tile);
 }
 
 // ---------------------------------------------------------------------------
 // P2
 // ---------------------------------------------------------------------------
 template <typename T, int NX>
-__global__ void __launch_bounds__(kTile * TeamShape<NX>::TT)
+__global__ void __launch_bounds__(kTile * TeamShape<NX>::TT, (waves_per_eu<T, NX>()))
 k_p2(StepArgs<T> a) {
     using S = TeamShape<NX>;
     constexpr int P = S::P, TT = S::TT, L = lds_len<NX>();
@@ -151,45 +170,40 @@ k_p2(StepArgs<T> a) {
     const int nk = g.nkxh;
     cplx<T>* lds = sm + team * L;
     auto sync = [] { __syncthreads(); };
-    const int npairs = a.ph.linear ? 1 : 3;
 
-    T vx[P], vy[P], bw[P], br[P];
-#pragma unroll 1
-    for (int p = 0; p < npairs; ++p) {
-        const cplx<T>* GA = a.G + (long long)(2 * p) * g.I + (long long)tile * nk * 4;
+    // x-inverse of field pair PP (0: velocities, 1: omega derivatives, 2: rho
+    // derivatives) for this team's column: Z(kx) = A + iB for kx >= 0 and
+    // conj(A) + i conj(B) at nx - kx, assembled branch-free from the LDS tile.
+    auto pair_ifft = [&](auto pc, cplx<T>* v) {
+        constexpr int PP = decltype(pc)::value;
+        const cplx<T>* GA = a.G + (long long)(2 * PP) * g.I + (long long)tile * nk * 4;
         const cplx<T>* GB = GA + g.I;
         for (int idx = threadIdx.x; idx < nk * 4; idx += blockDim.x) {
             sm[idx] = GA[idx];
             sm[nk * 4 + idx] = GB[idx];
         }
         __syncthreads();
-        cplx<T> v[P];
 #pragma unroll
         for (int m = 0; m < P; ++m) {
             const int e = tid + m * TT;
-            cplx<T> z = czero<T>();
-            if (e < nk) {
-                const cplx<T> A = sm[e * 4 + team], B = sm[nk * 4 + e * 4 + team];
-                z = e == 0 ? cplx<T>{A.x, B.x} : cplx<T>{A.x - B.y, A.y + B.x};
-            } else if (e > NX - nk) {
-                const int kk = NX - e;
-                const cplx<T> A = sm[kk * 4 + team], B = sm[nk * 4 + kk * 4 + team];
-                z = {A.x + B.y, B.x - A.y};
-            }
-            v[m] = z;
+            const bool pos = e < nk;
+            const bool keep = pos || e > NX - nk;
+            const int kk = keep ? (pos ? e : NX - e) : 0;
+            const cplx<T> A = sm[kk * 4 + team], B = sm[nk * 4 + kk * 4 + team];
+            const cplx<T> z = pos ? cplx<T>{A.x - B.y, A.y + B.x} : cplx<T>{A.x + B.y, B.x - A.y};
+            v[m] = keep ? z : czero<T>();
         }
+        if (tid == 0) v[0] = {sm[team].x, sm[nk * 4 + team].x};   // kx = 0: real parts only
         __syncthreads();
         TeamFFT<T, NX, P, S::RMAX, +1>::run(v, lds, a.twx, tid, sync);
-        if (p == 0) {
+    };
+
+    T vx[P], vy[P];
+    {
+        cplx<T> v[P];
+        pair_ifft(std::integral_constant<int, 0>{}, v);
 #pragma unroll
-            for (int m = 0; m < P; ++m) { vx[m] = v[m].x; vy[m] = v[m].y; }
-        } else if (p == 1) {
-#pragma unroll
-            for (int m = 0; m < P; ++m) bw[m] = vx[m] * v[m].x + vy[m] * v[m].y;
-        } else {
-#pragma unroll
-            for (int m = 0; m < P; ++m) br[m] = vx[m] * v[m].x + vy[m] * v[m].y;
-        }
+        for (int m = 0; m < P; ++m) { vx[m] = v[m].x; vy[m] = v[m].y; }
     }
 
     // CFL maxima (add_v0 + pl_max_search): max|vx + sigma*yy[y]|, max|vy|
@@ -213,10 +227,20 @@ k_p2(StepArgs<T> a) {
     }
     if (a.ph.linear) return;
 
-    // x-forward of z = -B_omega - i B_rho, then split into the two spectra
-    cplx<T> v[P];
+    // brackets B = vx*df/dx + vy*df/dy (calc_poisson_bracket), negated (negative)
+    T bw[P];
+    {
+        cplx<T> v[P];
+        pair_ifft(std::integral_constant<int, 1>{}, v);
 #pragma unroll
-    for (int m = 0; m < P; ++m) v[m] = {-bw[m], -br[m]};
+        for (int m = 0; m < P; ++m) bw[m] = vx[m] * v[m].x + vy[m] * v[m].y;
+    }
+    cplx<T> v[P];
+    pair_ifft(std::integral_constant<int, 2>{}, v);
+#pragma unroll
+    for (int m = 0; m < P; ++m) v[m] = {-bw[m], -(vx[m] * v[m].x + vy[m] * v[m].y)};
+
+    // x-forward of z = -B_omega - i B_rho, then split into the two spectra
     TeamFFT<T, NX, P, S::RMAX, -1>::run(v, lds, a.twx, tid, sync);
 #pragma unroll
     for (int m = 0; m < P; ++m) lds[tid + m * TT] = v[m];
@@ -315,42 +339,46 @@ __device__ void p3_bookkeeping(const StepArgs<T>& a, double dt, int h, int which
         a.s_out[i] = sn;
     }
     if (threadIdx.x == 0) {
+        // field by field into ctl_out (no private copy with dynamically indexed arrays)
         const long long n = c->istep;
-        StepCtl o = *c;
-        o.time = __dadd_rn(c->time, dt);
-        o.delt = dt;
-        o.istep = n + 1;
-        o.epoch = newE;
+        StepCtl* o = a.ctl_out;
+        o->time = __dadd_rn(c->time, dt);
+        o->delt = dt;
+        o->istep = n + 1;
+        o->epoch = newE;
         const int s0 = slot_a0(n), s1 = slot_a1(n);
-        if (trig && (c->slot_epoch_a[s1] != c->epoch || c->slot_epoch_d[s1] != c->epoch)) o.error |= 1;
-        o.slot_epoch_a[s0] = c->epoch;
-        o.slot_epoch_d[s0] = c->epoch;
-        if (a.store_phi) o.phi_epoch = c->epoch;
-        o.remap_count = c->remap_count + trig;
-        if (trig) o.last_remap_step = n;
-        o.halvings = c->halvings + h;
+        o->error = c->error | ((trig && (c->slot_epoch_a[s1] != c->epoch || c->slot_epoch_d[s1] != c->epoch)) ? 1 : 0);
+        for (int s = 0; s < 3; ++s) {
+            o->slot_epoch_a[s] = s == s0 ? c->epoch : c->slot_epoch_a[s];
+            o->slot_epoch_d[s] = s == s0 ? c->epoch : c->slot_epoch_d[s];
+        }
+        o->phi_epoch = a.store_phi ? c->epoch : c->phi_epoch;
+        o->remap_count = c->remap_count + trig;
+        o->last_remap_step = trig ? n : c->last_remap_step;
+        o->halvings = c->halvings + h;
+        o->log_count = c->log_count;
         if (h > 0 && a.hlog != nullptr && c->log_count < a.hlog_cap) {
-            const double vals[4] = {__longlong_as_double((long long)c->cfl_bits[0]),
-                                    __longlong_as_double((long long)c->cfl_bits[1]), a.ph.g / a.ph.rho0,
-                                    a.ph.rho0_prime};
+            const double value = which == 0 ? __longlong_as_double((long long)c->cfl_bits[0])
+                               : which == 1 ? __longlong_as_double((long long)c->cfl_bits[1])
+                               : which == 2 ? a.ph.g / a.ph.rho0
+                               : which == 3 ? a.ph.rho0_prime : 0.0;
             HalvingRec rec;
             rec.istep = n;
             rec.time = c->time;
-            rec.value = which >= 0 ? vals[which] : 0.0;
+            rec.value = value;
             rec.delt = dt;
             rec.which = which;
             rec.pad_ = h;
             a.hlog[c->log_count] = rec;
-            o.log_count = c->log_count + 1;
+            o->log_count = c->log_count + 1;
         }
-        o.cfl_bits[0] = 0ull;
-        o.cfl_bits[1] = 0ull;
-        *a.ctl_out = o;
+        o->cfl_bits[0] = 0ull;
+        o->cfl_bits[1] = 0ull;
     }
 }
 
 template <typename T, int NY>
-__global__ void __launch_bounds__(kTile * TeamShape<NY>::TT)
+__global__ void __launch_bounds__(kTile * TeamShape<NY>::TT, (waves_per_eu<T, NY>()))
 k_p3(StepArgs<T> a) {
     using S = TeamShape<NY>;
     constexpr int P = S::P, TT = S::TT, L = lds_len<NY>();
@@ -358,7 +386,8 @@ k_p3(StepArgs<T> a) {
     const Geo& g = a.g;
     const Phys& ph = a.ph;
     const int team = threadIdx.x / TT, tid = threadIdx.x - team * TT;
-    const int tile = blockIdx.x;
+    const int f = blockIdx.x / g.ntile_x;   // 0: omega, 1: rho
+    const int tile = blockIdx.x - f * g.ntile_x;
     const int row = tile * kTile + team;
     const bool act = row < g.nkxh;
     const int rn = row == 0 ? 0 : g.nkx - row;
@@ -382,8 +411,7 @@ k_p3(StepArgs<T> a) {
     const int ist = n == 0 ? 0 : (n == 1 ? 1 : 2);
     const T scale = (T)g.scale;
 
-#pragma unroll 1
-    for (int f = 0; f < 2; ++f) {
+    {
         if (!ph.linear) {
             const cplx<T>* Hf = a.H + (long long)f * g.I;
             for (int idx = threadIdx.x; idx < NY * 4; idx += blockDim.x) {
@@ -393,10 +421,7 @@ k_p3(StepArgs<T> a) {
             __syncthreads();
             cplx<T> v[P];
 #pragma unroll
-            for (int m = 0; m < P; ++m) {
-                const int e = tid + m * TT;
-                v[m] = sm[(e >> 2) * 16 + team * 4 + (e & 3)];
-            }
+            for (int m = 0; m < P; ++m) v[m] = sm[stage_index(tid, team) + 4 * TT * m];
             __syncthreads();
             TeamFFT<T, NY, P, S::RMAX, -1>::run(v, lds, a.twy, tid, sync);
 #pragma unroll
