@@ -224,6 +224,23 @@ struct TeamFFT {
     }
 };
 
+// Ordering of a team's LDS exchange when the team is one wavefront: the lanes
+// of a wave issue their LDS operations in program order, so only the compiler
+// must be kept from moving loads across the stores.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Team geometry with an explicit point count per thread.
+template <int N, int PTS>
+struct Team {
+    static constexpr int P = PTS;
+    static constexpr int TT = N / PTS;
+    static constexpr int RMAX = 16;
+};
+
 // Team geometry for a transform length: P points per thread, TT threads.
 template <int N>
 struct TeamShape {

@@ -73,7 +73,7 @@ template <>
 hipError_t Launch<R>::p2(const StepArgs<R>& a, int grid, hipStream_t s) {
     bool ok = dispatch(a.g.nx, [&](auto nc) {
         constexpr int N = decltype(nc)::value;
-        hipLaunchKernelGGL((k_p2<R, N>), dim3(grid), dim3(kTile * TeamShape<N>::TT), 0, s, a);
+        hipLaunchKernelGGL((k_p2<R, N>), dim3(grid), dim3(kTile * Team<N, p2_points<R, N>()>::TT), 0, s, a);
     });
     return ok ? hipGetLastError() : unsupported();
 }

@@ -23,14 +23,29 @@ constexpr int kTile = 4;   // rows (kx) per P1/P3 workgroup == columns (y) per P
 // Waves per SIMD that the LDS of a kTile-team workgroup allows (160 KiB per
 // CU, four SIMDs), capped at 4 (128 VGPRs).  Used as the occupancy target
 // of the transform kernels so the compiler keeps their registers within it.
-template <typename T, int N>
+template <typename T, int N, int TT = TeamShape<N>::TT>
 constexpr int waves_per_eu() {
     constexpr long long lds = (long long)kTile * lds_len<N>() * (long long)sizeof(cplx<T>);
     constexpr long long wgs = (160LL * 1024LL) / lds;
-    constexpr long long waves = wgs * ((kTile * TeamShape<N>::TT + 63) / 64);
+    constexpr long long waves = wgs * ((kTile * TT + 63) / 64);
     constexpr long long w = waves / 4;
     return w < 1 ? 1 : (w > 4 ? 4 : (int)w);
 }
+
+// Points per thread of each step kernel.
+template <typename T, int N> constexpr int p1_points() { return TeamShape<N>::P; }
+template <typename T, int N> constexpr int p2_points() { return TeamShape<N>::P; }
+template <typename T, int N> constexpr int p3_points() { return TeamShape<N>::P; }
+
+// Exchange synchronisation of a team: a wave-level fence for one-wave teams,
+// otherwise a workgroup barrier.
+template <int TT>
+struct TeamSync {
+    __device__ __forceinline__ void operator()() const {
+        if constexpr (TT == 64) wave_sync();
+        else __syncthreads();
+    }
+};
 
 struct Geo {
     int nx, ny, nkx, nky, nkxh, nkxh2, nkxpad, ncy;
@@ -62,7 +77,6 @@ struct StepCtl {
     int error;                         // bit 0: remap while a live slot was still pending
     int log_count;
     long long last_remap_step;
-    unsigned long long cfl_bits[2];    // max|vx + sigma*y|, max|vy| as double bits (P2 atomics)
 };
 
 struct HalvingRec {
@@ -94,7 +108,36 @@ struct StepArgs {
     HalvingRec* hlog;
     int hlog_cap;
     int store_phi;
+    double* cflp;   // [ntile_y][2]: per-workgroup max|vx + sigma*y|, max|vy| of P2
 };
+
+// Wavefront max of a double.
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+    return v;
+}
+
+// Reduce P2's per-workgroup CFL maxima over the whole grid (every P3
+// workgroup does this; `red` is workgroup-shared scratch of 2*16 doubles).
+__device__ __forceinline__ void cfl_maxima(const double* __restrict__ cflp, int n, double* red, double* cvx,
+                                           double* cvy) {
+    double m0 = 0.0, m1 = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        m0 = fmax(m0, cflp[2 * i]);
+        m1 = fmax(m1, cflp[2 * i + 1]);
+    }
+    m0 = wave_max(m0);
+    m1 = wave_max(m1);
+    const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    if ((threadIdx.x & 63) == 0) { red[w] = m0; red[16 + w] = m1; }
+    __syncthreads();
+    m0 = 0.0;
+    m1 = 0.0;
+    for (int i = 0; i < nw; ++i) { m0 = fmax(m0, red[i]); m1 = fmax(m1, red[16 + i]); }
+    *cvx = m0;
+    *cvy = m1;
+}
 
 // Ring slots after the renew_fields rotation of step n (time_integral.cu:474-484):
 // a0 (written) = (-(n+1)) mod 3, a1 (state n) = (-n) mod 3, a2 = (1-n) mod 3.
@@ -138,11 +181,9 @@ __device__ __forceinline__ double k2_of(double kx, double kys) {
 
 // The step size after the CFL halvings of check_cfl (time_integral.cu:357-383).
 // Returns the halving count; `which` gets the last condition that halved.
-__device__ __forceinline__ int cfl_delt(const StepCtl* c, const Geo& g, const Phys& ph, double* dt_out,
-                                        int* which) {
+__device__ __forceinline__ int cfl_delt(const StepCtl* c, const Geo& g, const Phys& ph, double cvx, double cvy,
+                                        double* dt_out, int* which) {
     double dt = c->delt;
-    const double cvx = __longlong_as_double((long long)c->cfl_bits[0]);
-    const double cvy = __longlong_as_double((long long)c->cfl_bits[1]);
     const double cg = ph.g / ph.rho0;
     const double cr = ph.rho0_prime;
     int h = 0;

@@ -78,7 +78,7 @@ __device__ __forceinline__ void p1_body(const StepArgs<T>& a, cplx<T>* sm, int t
     const cplx<T>* src = GRP == 2 ? a.a_rho[s1] : a.a_omg[s1];
     const int* jmp = (c->slot_epoch_a[s1] != c->epoch) ? a.jump + (c->epoch & 1) * g.nkx : nullptr;
     cplx<T>* lds = sm + team * L;
-    auto sync = [] { __syncthreads(); };
+    const TeamSync<S::TT> sync{};
     const double kxp = a.kx[rowc], kxn = a.kx[rn];
     const double sp = a.s_in[rowc], sn = a.s_in[rn];
     const int nrow_tile = min(kTile, g.nkxh - tile * kTile);
@@ -157,9 +157,10 @@ k_p1(StepArgs<T> a) {
 // P2
 // ---------------------------------------------------------------------------
 template <typename T, int NX>
-__global__ void __launch_bounds__(kTile * TeamShape<NX>::TT, (waves_per_eu<T, NX>()))
+__global__ void __launch_bounds__((kTile * Team<NX, p2_points<T, NX>()>::TT),
+                                  (waves_per_eu<T, NX, Team<NX, p2_points<T, NX>()>::TT>()))
 k_p2(StepArgs<T> a) {
-    using S = TeamShape<NX>;
+    using S = Team<NX, p2_points<T, NX>()>;
     constexpr int P = S::P, TT = S::TT, L = lds_len<NX>();
     constexpr int NQ = (NX / 3 + 1 + TT - 1) / TT;   // >= ceil(nkxh / TT)
     __shared__ cplx<T> sm[kTile * L];
@@ -169,7 +170,7 @@ k_p2(StepArgs<T> a) {
     const int y = tile * kTile + team;
     const int nk = g.nkxh;
     cplx<T>* lds = sm + team * L;
-    auto sync = [] { __syncthreads(); };
+    const TeamSync<S::TT> sync{};
 
     // x-inverse of field pair PP (0: velocities, 1: omega derivatives, 2: rho
     // derivatives) for this team's column: Z(kx) = A + iB for kx >= 0 and
@@ -215,14 +216,17 @@ k_p2(StepArgs<T> a) {
             m0 = fmax(m0, fabs(__dadd_rn((double)vx[m], sy)));
             m1 = fmax(m1, fabs((double)vy[m]));
         }
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            m0 = fmax(m0, __shfl_xor(m0, off));
-            m1 = fmax(m1, __shfl_xor(m1, off));
-        }
-        if ((threadIdx.x & 63) == 0) {
-            atomicMax(&a.ctl_in->cfl_bits[0], (unsigned long long)__double_as_longlong(m0));
-            atomicMax(&a.ctl_in->cfl_bits[1], (unsigned long long)__double_as_longlong(m1));
+        // one partial pair per workgroup, no atomics (P3 reduces the array)
+        __shared__ double red[32];
+        m0 = wave_max(m0);
+        m1 = wave_max(m1);
+        const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+        if ((threadIdx.x & 63) == 0) { red[w] = m0; red[16 + w] = m1; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int i = 1; i < nw; ++i) { m0 = fmax(m0, red[i]); m1 = fmax(m1, red[16 + i]); }
+            a.cflp[2 * blockIdx.x] = m0;
+            a.cflp[2 * blockIdx.x + 1] = m1;
         }
     }
     if (a.ph.linear) return;
@@ -321,7 +325,7 @@ __device__ __forceinline__ cplx<T> ab_bdf_update(int ist, bool plain, double dt,
 // Bookkeeping of step n done by workgroup 0 of P3: the shift advance and remap
 // trigger of update_shear (shear.cu:211-294) in FP64, and the next control block.
 template <typename T>
-__device__ void p3_bookkeeping(const StepArgs<T>& a, double dt, int h, int which) {
+__device__ void p3_bookkeeping(const StepArgs<T>& a, double dt, int h, int which, double cvx, double cvy) {
     const Geo& g = a.g;
     const StepCtl* c = a.ctl_in;
     const double sdt1 = g.nkx > 1 ? __dmul_rn(__dmul_rn(a.ph.sigma, a.kx[1]), dt) : 0.0;
@@ -358,8 +362,8 @@ __device__ void p3_bookkeeping(const StepArgs<T>& a, double dt, int h, int which
         o->halvings = c->halvings + h;
         o->log_count = c->log_count;
         if (h > 0 && a.hlog != nullptr && c->log_count < a.hlog_cap) {
-            const double value = which == 0 ? __longlong_as_double((long long)c->cfl_bits[0])
-                               : which == 1 ? __longlong_as_double((long long)c->cfl_bits[1])
+            const double value = which == 0 ? cvx
+                               : which == 1 ? cvy
                                : which == 2 ? a.ph.g / a.ph.rho0
                                : which == 3 ? a.ph.rho0_prime : 0.0;
             HalvingRec rec;
@@ -372,8 +376,6 @@ __device__ void p3_bookkeeping(const StepArgs<T>& a, double dt, int h, int which
             a.hlog[c->log_count] = rec;
             o->log_count = c->log_count + 1;
         }
-        o->cfl_bits[0] = 0ull;
-        o->cfl_bits[1] = 0ull;
     }
 }
 
@@ -395,12 +397,15 @@ k_p3(StepArgs<T> a) {
     const StepCtl* c = a.ctl_in;
     const long long n = c->istep;
     cplx<T>* lds = sm + team * L;
-    auto sync = [] { __syncthreads(); };
+    const TeamSync<S::TT> sync{};
 
     double dt;
     int which;
-    const int h = cfl_delt(c, g, ph, &dt, &which);
-    if (blockIdx.x == 0) p3_bookkeeping(a, dt, h, which);
+    __shared__ double red[32];
+    double cvx, cvy;
+    cfl_maxima(a.cflp, g.ntile_y, red, &cvx, &cvy);
+    const int h = cfl_delt(c, g, ph, cvx, cvy, &dt, &which);
+    if (blockIdx.x == 0) p3_bookkeeping(a, dt, h, which, cvx, cvy);
 
     const int s0 = slot_a0(n), s1 = slot_a1(n), s2 = slot_a2(n);
     const int* jbase = a.jump + (c->epoch & 1) * g.nkx;

@@ -14,7 +14,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libshrt_hip.so")
+LIB_PATH = os.environ.get("SHRT_LIB", os.path.join(HERE, "libshrt_hip.so"))
 REPO = os.path.dirname(HERE)
 HEADER = os.path.join(REPO, "include", "shrt.h")
 
