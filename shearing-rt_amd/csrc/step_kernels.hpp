@@ -63,9 +63,11 @@ __device__ __forceinline__ cplx<T> p1_mult(const ModeK<T>& k, bool origin, cplx<
     }
 }
 
-// One P1 workgroup: kTile row pairs (kx, -kx) of group GRP, two outputs each.
-template <int GRP, typename T, int NY>
-__device__ __forceinline__ void p1_body(const StepArgs<T>& a, cplx<T>* sm, int team, int tid, int tile) {
+// One output OI (0..5) of P1 for a tile of kTile row pairs (kx, -kx): group
+// GRP = OI/2 (0: phi -> vx, vy; 1: omega; 2: rho), output O = OI%2.
+template <int OI, typename T, int NY>
+__device__ __forceinline__ void p1_output(const StepArgs<T>& a, cplx<T>* sm, int team, int tid, int tile) {
+    constexpr int GRP = OI / 2, O = OI % 2;
     using S = TeamShape<NY>;
     constexpr int P = S::P, TT = S::TT, L = lds_len<NY>();
     const Geo& g = a.g;
@@ -83,47 +85,44 @@ __device__ __forceinline__ void p1_body(const StepArgs<T>& a, cplx<T>* sm, int t
     const double sp = a.s_in[rowc], sn = a.s_in[rn];
     const int nrow_tile = min(kTile, g.nkxh - tile * kTile);
 
-    static_for<0, 2>([&](auto oc) {
-        constexpr int O = decltype(oc)::value;
-        // y-spectrum of the row pair in natural order, branch-free: element e
-        // takes row kx at iky = e (e < nky) or the conjugate of row -kx at
-        // iky = NY - e (e > NY - nky); padding lanes load an in-bounds element
-        // and are zeroed.
-        cplx<T> v[P];
+    // y-spectrum of the row pair in natural order, branch-free: element e
+    // takes row kx at iky = e (e < nky) or the conjugate of row -kx at
+    // iky = NY - e (e > NY - nky); padding lanes load an in-bounds element
+    // and are zeroed.
+    cplx<T> v[P];
 #pragma unroll
-        for (int m = 0; m < P; ++m) {
-            const int e = tid + m * TT;
-            const bool pos = e < g.nky;
-            const bool keep = act && (pos || e > NY - g.nky);
-            const int iky = keep ? (pos ? e : NY - e) : 0;
-            const int r = pos ? rowc : rn;
-            const cplx<T> F = view_load(src, r, iky, g, jmp);
-            const ModeK<T> k = mode_k<T>(a.ky, pos ? sp : sn, pos ? kxp : kxn, iky);
-            const cplx<T> z = p1_mult<GRP, O, T>(k, r == 0 && iky == 0, F);
-            v[m] = keep ? (pos ? z : cplx<T>{z.x, -z.y}) : czero<T>();
-        }
-        if (act && tid == 0) {
-            // Hermitian part of the ky=0 column, as the x-first 2-D C2R uses it.
-            const ModeK<T> k = mode_k<T>(a.ky, sn, kxn, 0);
-            const cplx<T> vn = p1_mult<GRP, O, T>(k, rn == 0, view_load(src, rn, 0, g, jmp));
-            v[0] = {(v[0].x + vn.x) * T(0.5), (v[0].y - vn.y) * T(0.5)};
-        }
-        TeamFFT<T, NY, P, S::RMAX, +1>::run(v, lds, a.twy, tid, sync);
-        // stage as [t][team][c] so each 4x4 block is one contiguous run of G
-        const int st = stage_index(tid, team);
+    for (int m = 0; m < P; ++m) {
+        const int e = tid + m * TT;
+        const bool pos = e < g.nky;
+        const bool keep = act && (pos || e > NY - g.nky);
+        const int iky = keep ? (pos ? e : NY - e) : 0;
+        const int r = pos ? rowc : rn;
+        const cplx<T> F = view_load(src, r, iky, g, jmp);
+        const ModeK<T> k = mode_k<T>(a.ky, pos ? sp : sn, pos ? kxp : kxn, iky);
+        const cplx<T> z = p1_mult<GRP, O, T>(k, r == 0 && iky == 0, F);
+        v[m] = keep ? (pos ? z : cplx<T>{z.x, -z.y}) : czero<T>();
+    }
+    if (act && tid == 0) {
+        // Hermitian part of the ky=0 column, as the x-first 2-D C2R uses it.
+        const ModeK<T> k = mode_k<T>(a.ky, sn, kxn, 0);
+        const cplx<T> vn = p1_mult<GRP, O, T>(k, rn == 0, view_load(src, rn, 0, g, jmp));
+        v[0] = {(v[0].x + vn.x) * T(0.5), (v[0].y - vn.y) * T(0.5)};
+    }
+    TeamFFT<T, NY, P, S::RMAX, +1>::run(v, lds, a.twy, tid, sync);
+    // stage as [t][team][c] so each 4x4 block is one contiguous run of G
+    const int st = stage_index(tid, team);
 #pragma unroll
-        for (int m = 0; m < P; ++m) sm[st + 4 * TT * m] = v[m];
-        __syncthreads();
-        cplx<T>* Gf = a.G + (long long)(GRP * 2 + O) * g.I;
-        for (int idx = threadIdx.x; idx < NY * 4; idx += blockDim.x) {
-            const int t = idx >> 4, w = idx & 15;
-            if ((w >> 2) < nrow_tile) Gf[((long long)t * g.nkxh + tile * kTile) * 4 + w] = sm[idx];
-        }
-        __syncthreads();
-    });
+    for (int m = 0; m < P; ++m) sm[st + 4 * TT * m] = v[m];
+    __syncthreads();
+    cplx<T>* Gf = a.G + (long long)OI * g.I;
+    for (int idx = threadIdx.x; idx < NY * 4; idx += blockDim.x) {
+        const int t = idx >> 4, w = idx & 15;
+        if ((w >> 2) < nrow_tile) Gf[((long long)t * g.nkxh + tile * kTile) * 4 + w] = sm[idx];
+    }
+    __syncthreads();
 
     // aphi (shear.cu:367-374) for the rows of this tile, when requested
-    if constexpr (GRP == 0) {
+    if constexpr (OI == 0) {
         if (a.store_phi && act) {
             for (int iky = tid; iky < g.nky; iky += TT) {
 #pragma unroll
@@ -140,17 +139,23 @@ __device__ __forceinline__ void p1_body(const StepArgs<T>& a, cplx<T>* sm, int t
     }
 }
 
+// Workgroups of P1: ntile_x row tiles times 6/OPW output chunks; workgroup
+// (tile, chunk) produces outputs chunk*OPW .. chunk*OPW + OPW - 1 (only 0 and 1
+// in the linear mode).
 template <typename T, int NY>
 __global__ void __launch_bounds__(kTile * TeamShape<NY>::TT, (waves_per_eu<T, NY>()))
 k_p1(StepArgs<T> a) {
     using S = TeamShape<NY>;
+    constexpr int OPW = p1_outputs_per_wg<T, NY>();
     __shared__ cplx<T> sm[kTile * lds_len<NY>()];
     const int team = threadIdx.x / S::TT, tid = threadIdx.x - team * S::TT;
     const int tile = blockIdx.x % a.g.ntile_x;
-    const int grp = blockIdx.x / a.g.ntile_x;   // 0: phi -> (vx, vy); 1: omega; 2: rho
-    if (grp == 0) p1_body<0, T, NY>(a, sm, team, tid, tile);
-    else if (grp == 1) p1_body<1, T, NY>(a, sm, team, tid, tile);
-    else p1_body<2, T, NY>(a, sm, team, tid, tile);
+    const int first = (blockIdx.x / a.g.ntile_x) * OPW;
+    const int last = a.ph.linear ? 2 : 6;
+    static_for<0, 6>([&](auto oc) {
+        constexpr int OI = decltype(oc)::value;
+        if (OI >= first && OI < first + OPW && OI < last) p1_output<OI, T, NY>(a, sm, team, tid, tile);
+    });
 }
 
 // ---------------------------------------------------------------------------
