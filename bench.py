@@ -61,10 +61,48 @@ def read_traffic(name):
     if not os.path.exists(path):
         return None
     try:
-        d = json.load(open(path))
-        return d.get(name)
+        d = json.load(open(path)).get(name)
+        return None if d is None else d["bytes"]
     except Exception:
         return None
+
+
+def pmc_summary(fetch_csv, write_csv, cal_fetch_csv, cal_write_csv, cal_bytes):
+    """HBM bytes per dispatch of each step kernel from rocprofv3 --pmc runs.
+
+    FETCH_SIZE and WRITE_SIZE (KiB per dispatch) are scaled by what they read
+    for the plain copy kernel k_copy16 moving `cal_bytes` (read once, written
+    once).  Returns {"p1_<n>_<fp32|fp64>": bytes, ..., "calibration": {...}}.
+    """
+    import collections
+    import csv
+    import re
+
+    def per_kernel(path):
+        acc = collections.defaultdict(list)
+        for r in csv.DictReader(open(path)):
+            acc[r["Kernel_Name"]].append(float(r["Counter_Value"]))
+        return {k: sum(v) / len(v) for k, v in acc.items()}
+
+    def copy_kib(d):
+        v = [x for k, x in d.items() if "k_copy16" in k]
+        if not v:
+            raise RuntimeError("calibration run holds no k_copy16 dispatch")
+        return v[0]
+
+    f_cal = cal_bytes / (copy_kib(per_kernel(cal_fetch_csv)) * 1024.0)
+    w_cal = cal_bytes / (copy_kib(per_kernel(cal_write_csv)) * 1024.0)
+    fetch, write = per_kernel(fetch_csv), per_kernel(write_csv)
+    out = {"calibration": {"copy_bytes": cal_bytes, "fetch_factor": f_cal, "write_factor": w_cal}}
+    pat = re.compile(r"k_(p[123])<(float|double), (\d+)>")
+    for k, fv in fetch.items():
+        m = pat.search(k)
+        if not m or k not in write:
+            continue
+        key = f"{m.group(1)}_{m.group(3)}_{'fp32' if m.group(2) == 'float' else 'fp64'}"
+        out[key] = {"bytes": f_cal * fv * 1024.0 + w_cal * write[k] * 1024.0,
+                    "read_bytes": f_cal * fv * 1024.0, "write_bytes": w_cal * write[k] * 1024.0}
+    return out
 
 
 def cpu_baseline(orc, n, params, ic, budget_s):
@@ -75,7 +113,7 @@ def cpu_baseline(orc, n, params, ic, budget_s):
     t0 = time.perf_counter()
     o.step(1)
     one = time.perf_counter() - t0
-    steps = int(max(2, min(50, budget_s / max(one, 1e-6))))
+    steps = int(max(2, min(1000, budget_s / max(one, 1e-6))))
     t0 = time.perf_counter()
     o.step(steps)
     dt = time.perf_counter() - t0
@@ -98,7 +136,16 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel on BYTES (the HBM-counter reference) and exit")
+    ap.add_argument("--pmc-summary", nargs=5, metavar=("FETCH_CSV", "WRITE_CSV", "CAL_FETCH_CSV",
+                                                       "CAL_WRITE_CSV", "CAL_BYTES"),
+                    help="only turn rocprofv3 FETCH_SIZE/WRITE_SIZE counter files into HBM bytes per "
+                         "dispatch (JSON on stdout) and exit")
     args = ap.parse_args()
+
+    if args.pmc_summary:
+        f, w, cf, cw, nb = args.pmc_summary
+        print(json.dumps(pmc_summary(f, w, cf, cw, int(nb)), indent=1), flush=True)
+        return
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:

@@ -81,14 +81,16 @@ k_aux_yfwd(AuxArgs<T> a) {
     cplx<T>* F = static_cast<cplx<T>*>(a.out);
     cplx<T>* lds = sm + team * L;
     auto sync = [] { __syncthreads(); };
+    constexpr int LP = stage_stride<NY>();
     for (int idx = threadIdx.x; idx < NY * 4; idx += blockDim.x) {
         const int t = idx >> 4, w = idx & 15;
-        sm[idx] = (w >> 2) < nrow_tile ? Hf[((long long)t * g.nkxh + tile * kTile) * 4 + w] : czero<T>();
+        sm[(w >> 2) * LP + 4 * t + (w & 3)] =
+            (w >> 2) < nrow_tile ? Hf[((long long)t * g.nkxh + tile * kTile) * 4 + w] : czero<T>();
     }
     __syncthreads();
     cplx<T> v[P];
 #pragma unroll
-    for (int m = 0; m < P; ++m) v[m] = sm[stage_index(tid, team) + 4 * TT * m];
+    for (int m = 0; m < P; ++m) v[m] = sm[team * LP + tid + m * TT];
     __syncthreads();
     TeamFFT<T, NY, P, S::RMAX, -1>::run(v, lds, a.tw, tid, sync);
 #pragma unroll
@@ -155,13 +157,15 @@ k_aux_yinv(AuxArgs<T> a) {
         v[m] = val;
     }
     TeamFFT<T, NY, P, S::RMAX, +1>::run(v, lds, a.tw, tid, sync);
+    constexpr int LP = stage_stride<NY>();
 #pragma unroll
-    for (int m = 0; m < P; ++m) sm[stage_index(tid, team) + 4 * TT * m] = v[m];
+    for (int m = 0; m < P; ++m) sm[team * LP + tid + m * TT] = v[m];
     __syncthreads();
     cplx<T>* Gf = static_cast<cplx<T>*>(a.out);
     for (int idx = threadIdx.x; idx < NY * 4; idx += blockDim.x) {
         const int t = idx >> 4, w = idx & 15;
-        if ((w >> 2) < nrow_tile) Gf[((long long)t * g.nkxh + tile * kTile) * 4 + w] = sm[idx];
+        if ((w >> 2) < nrow_tile)
+            Gf[((long long)t * g.nkxh + tile * kTile) * 4 + w] = sm[(w >> 2) * LP + 4 * t + (w & 3)];
     }
 }
 

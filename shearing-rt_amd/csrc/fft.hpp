@@ -133,6 +133,47 @@ __device__ __forceinline__ void dft(cplx<T>* v) {
 __device__ __forceinline__ int lpad(int i) { return i + (i >> 5); }
 template <int N> constexpr int lds_len() { return N + N / 32; }
 
+// Radix and butterflies per thread of the stage that starts at sub-length NS.
+template <int N, int P, int RMAX, int NS>
+struct StageShape {
+    static constexpr int REM = N / NS;
+    static constexpr int R0 = REM < RMAX ? REM : RMAX;
+    static constexpr int R = R0 < P ? R0 : P;
+    static constexpr int Q = P / R;
+    static constexpr bool LAST = (NS * R == N);
+};
+
+// The table twiddles of every stage of one team transform, held in registers:
+// w_1 (and w_4 for radix >= 8) per butterfly, W_{NS*R}^{k} = tw[k*N/(NS*R)] for
+// the butterfly's position k.  Loading them before a transform keeps global
+// loads out of the transform, so loads a caller issues ahead of it stay in
+// flight (vector-memory loads complete in order).  The same values serve the
+// forward and the inverse transform.
+template <typename T, int N, int P, int RMAX, int NS = 1, bool MORE = (NS < N)>
+struct TwiddleSet {
+    using Sh = StageShape<N, P, RMAX, NS>;
+    static constexpr int TT = N / P;
+    cplx<T> w1[Sh::Q], w4[Sh::Q];
+    TwiddleSet<T, N, P, RMAX, NS * Sh::R> next;
+
+    __device__ __forceinline__ void load(const cplx<T>* __restrict__ tw, int tid) {
+        if constexpr (NS > 1) {
+            constexpr int UNIT = N / (NS * Sh::R);
+#pragma unroll
+            for (int q = 0; q < Sh::Q; ++q) {
+                const int k = (tid + q * TT) & (NS - 1);
+                w1[q] = tw[k * UNIT];
+                if constexpr (Sh::R >= 8) w4[q] = tw[4 * k * UNIT];
+            }
+        }
+        next.load(tw, tid);
+    }
+};
+template <typename T, int N, int P, int RMAX, int NS>
+struct TwiddleSet<T, N, P, RMAX, NS, false> {
+    __device__ __forceinline__ void load(const cplx<T>*, int) {}
+};
+
 // Team FFT of length N, P points per thread (TT = N/P threads), radix <= RMAX.
 // D = -1 forward (e^{-i}), +1 inverse; unnormalised.  `tw` holds W_N^k = e^{-2 pi i k/N}.
 // `lds` is the team's private buffer of lds_len<N>() elements.  SYNC is called
@@ -141,15 +182,13 @@ template <typename T, int N, int P, int RMAX, int D>
 struct TeamFFT {
     static constexpr int TT = N / P;
     static_assert(N % P == 0, "P must divide N");
+    using Twiddles = TwiddleSet<T, N, P, RMAX>;
 
-    template <int NS, typename SYNC>
-    __device__ static __forceinline__ void stages(cplx<T>* v, cplx<T>* lds, const cplx<T>* __restrict__ tw,
-                                                  int tid, SYNC&& sync) {
-        constexpr int REM = N / NS;
-        constexpr int R0 = REM < RMAX ? REM : RMAX;
-        constexpr int R = R0 < P ? R0 : P;
-        constexpr int Q = P / R;
-        constexpr bool LAST = (NS * R == N);
+    template <int NS, typename TS, typename SYNC>
+    __device__ static __forceinline__ void stages(cplx<T>* v, cplx<T>* lds, const TS& ts, int tid, SYNC&& sync) {
+        using Sh = StageShape<N, P, RMAX, NS>;
+        constexpr int R = Sh::R, Q = Sh::Q;
+        constexpr bool LAST = Sh::LAST;
         static_assert(P % R == 0, "stage radix must divide P");
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
@@ -163,8 +202,7 @@ struct TeamFFT {
                 // the others as short products of those (relative error <= ~8 ulp).
                 // Each power is applied as soon as it is formed; only w_1..w_4
                 // and the multiples of 4 stay live.
-                constexpr int UNIT = N / (NS * R);
-                cplx<T> w1 = tw[k * UNIT];
+                cplx<T> w1 = ts.w1[q];
                 if constexpr (D > 0) w1.y = -w1.y;
                 u[1] = cmul(u[1], w1);
                 if constexpr (R >= 4) {
@@ -173,7 +211,7 @@ struct TeamFFT {
                     u[2] = cmul(u[2], w2);
                     u[3] = cmul(u[3], w3);
                     if constexpr (R >= 8) {
-                        cplx<T> w4 = tw[4 * k * UNIT];
+                        cplx<T> w4 = ts.w4[q];
                         if constexpr (D > 0) w4.y = -w4.y;
                         cplx<T> w4m = w4;   // w_{4j}
 #pragma unroll
@@ -213,14 +251,22 @@ struct TeamFFT {
                 for (int m = 0; m < P; ++m) v[m] = lds[lpad(tid + m * TT)];
             }
             sync();
-            stages<NS * R>(v, lds, tw, tid, sync);
+            stages<NS * R>(v, lds, ts.next, tid, sync);
         }
+    }
+
+    // Transform with twiddles loaded by the caller beforehand.
+    template <typename SYNC>
+    __device__ static __forceinline__ void run(cplx<T>* v, cplx<T>* lds, const Twiddles& ts, int tid, SYNC&& sync) {
+        stages<1>(v, lds, ts, tid, sync);
     }
 
     template <typename SYNC>
     __device__ static __forceinline__ void run(cplx<T>* v, cplx<T>* lds, const cplx<T>* __restrict__ tw, int tid,
                                                SYNC&& sync) {
-        stages<1>(v, lds, tw, tid, sync);
+        Twiddles ts;
+        ts.load(tw, tid);
+        stages<1>(v, lds, ts, tid, sync);
     }
 };
 

@@ -51,6 +51,14 @@ struct TeamSync {
     }
 };
 
+// Join the teams of a workgroup before a buffer shared between teams is
+// rewritten.  Teams of several wavefronts already pass a workgroup barrier at
+// the end of every transform; one-wave teams synchronise only their own wave.
+template <int TT>
+__device__ __forceinline__ void teams_join() {
+    if constexpr (TT == 64) __syncthreads();
+}
+
 struct Geo {
     int nx, ny, nkx, nky, nkxh, nkxh2, nkxpad, ncy;
     int ntile_x;   // ceil(nkxh / kTile): row-pair tiles
@@ -151,10 +159,43 @@ __host__ __device__ inline int slot_a2(long long n) { return (int)(((1 - n) % 3 
 
 template <typename T> __device__ __forceinline__ cplx<T> czero() { return {T(0), T(0)}; }
 
-// Position of element e = tid + m*TT of team `team` in a workgroup staging
-// buffer laid out [e/4][team][e%4] (the G/H tile order): the thread part plus
-// 4*TT*m, valid because TT is a multiple of 4.
-__device__ __forceinline__ int stage_index(int tid, int team) { return (tid >> 2) * 16 + team * 4 + (tid & 3); }
+// Work item of workgroup b in a grid of n.  The dispatcher deals consecutive
+// workgroups to the eight XCDs in turn and each XCD has its own L2, so items
+// are renumbered to give XCD x one contiguous range of items, in dispatch
+// order: workgroups whose items share input rows then share an L2.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_item(int b, int n) {
+    const int x = b % kXcds, j = b / kXcds;
+    const int base = n / kXcds, rem = n % kXcds;
+    return x * base + min(x, rem) + j;
+}
+
+// A complex value over a real one: two IEEE divisions in FP64 (the reference's
+// cmplx / real operator), one division and two products in FP32 (within 1.5 ulp).
+template <typename T>
+__device__ __forceinline__ cplx<T> cdivr(cplx<T> z, T d) {
+    if constexpr (sizeof(T) == 8) {
+        return {z.x / d, z.y / d};
+    } else {
+        const float r = 1.0f / d;
+        return {z.x * r, z.y * r};
+    }
+}
+
+// Row stride of a workgroup staging buffer [team][y] between the team
+// transforms (natural order) and the [t][team][c] tiles of G and H.  N + 8
+// complex (== 8 mod 32) spreads the 16 (team, c) lanes of a tile row over
+// distinct LDS banks; it fits the transform buffers from N = 256 up.
+template <int N> constexpr int stage_stride() { return N >= 256 ? N + 8 : lds_len<N>(); }
+
+// Row stride of P2's staging buffer [field][c][kx] for two G fields of a
+// column tile: the kx extent rounded up to 32 plus 8 complex where that fits
+// the transform buffers.
+template <int NX> constexpr int p2_stage_stride() {
+    constexpr int nk = NX / 3 + 1;
+    constexpr int padded = (nk + 31) / 32 * 32 + 8;
+    return 8 * padded <= kTile * lds_len<NX>() ? padded : nk;
+}
 
 // Remapped read of a spectral buffer (shear.cu:296-322).  `jmp` is null when the
 // buffer is already in the current frame.

@@ -54,7 +54,7 @@ __device__ __forceinline__ ModeK<T> mode_k(const double* __restrict__ ky, double
 template <int GRP, int O, typename T>
 __device__ __forceinline__ cplx<T> p1_mult(const ModeK<T>& k, bool origin, cplx<T> F) {
     if constexpr (GRP == 0) {
-        const cplx<T> phi = origin ? czero<T>() : cplx<T>{F.x / k.k2, F.y / k.k2};
+        const cplx<T> phi = origin ? czero<T>() : cdivr(F, k.k2);
         if constexpr (O == 0) return {-k.kys * phi.y, k.kys * phi.x};
         else return {k.kx * phi.y, -k.kx * phi.x};
     } else {
@@ -109,15 +109,20 @@ __device__ __forceinline__ void p1_output(const StepArgs<T>& a, cplx<T>* sm, int
         v[0] = {(v[0].x + vn.x) * T(0.5), (v[0].y - vn.y) * T(0.5)};
     }
     TeamFFT<T, NY, P, S::RMAX, +1>::run(v, lds, a.twy, tid, sync);
-    // stage as [t][team][c] so each 4x4 block is one contiguous run of G
-    const int st = stage_index(tid, team);
+    // stage as [team][y], then write G tile by tile: 4x4 blocks [t][team][c]
+    // (y = 4t + c) are contiguous runs of G
+    constexpr int LP = stage_stride<NY>();
+    teams_join<TT>();
 #pragma unroll
-    for (int m = 0; m < P; ++m) sm[st + 4 * TT * m] = v[m];
+    for (int m = 0; m < P; ++m) sm[team * LP + tid + m * TT] = v[m];
     __syncthreads();
     cplx<T>* Gf = a.G + (long long)OI * g.I;
-    for (int idx = threadIdx.x; idx < NY * 4; idx += blockDim.x) {
+#pragma unroll
+    for (int m = 0; m < P; ++m) {   // NY*4 staged elements over kTile*TT threads
+        const int idx = threadIdx.x + m * kTile * TT;
         const int t = idx >> 4, w = idx & 15;
-        if ((w >> 2) < nrow_tile) Gf[((long long)t * g.nkxh + tile * kTile) * 4 + w] = sm[idx];
+        if ((w >> 2) < nrow_tile)
+            Gf[((long long)t * g.nkxh + tile * kTile) * 4 + w] = sm[(w >> 2) * LP + 4 * t + (w & 3)];
     }
     __syncthreads();
 
@@ -132,7 +137,7 @@ __device__ __forceinline__ void p1_output(const StepArgs<T>& a, cplx<T>* sm, int
                     const ModeK<T> k = mode_k<T>(a.ky, side ? sn : sp, side ? kxn : kxp, iky);
                     const cplx<T> F = view_load(src, r, iky, g, jmp);
                     a.aphi[(long long)r * g.nky + iky] =
-                        (r == 0 && iky == 0) ? czero<T>() : cplx<T>{F.x / k.k2, F.y / k.k2};
+                        (r == 0 && iky == 0) ? czero<T>() : cdivr(F, k.k2);
                 }
             }
         }
@@ -149,9 +154,12 @@ k_p1(StepArgs<T> a) {
     constexpr int OPW = p1_outputs_per_wg<T, NY>();
     __shared__ cplx<T> sm[kTile * lds_len<NY>()];
     const int team = threadIdx.x / S::TT, tid = threadIdx.x - team * S::TT;
-    const int tile = blockIdx.x % a.g.ntile_x;
-    const int first = (blockIdx.x / a.g.ntile_x) * OPW;
     const int last = a.ph.linear ? 2 : 6;
+    const int chunks = (last + OPW - 1) / OPW;
+    // the chunks of one tile read the same rows: keep them on one XCD
+    const int q = xcd_item(blockIdx.x, gridDim.x);
+    const int tile = q / chunks;
+    const int first = (q - tile * chunks) * OPW;
     static_for<0, 6>([&](auto oc) {
         constexpr int OI = decltype(oc)::value;
         if (OI >= first && OI < first + OPW && OI < last) p1_output<OI, T, NY>(a, sm, team, tid, tile);
@@ -167,8 +175,13 @@ __global__ void __launch_bounds__((kTile * Team<NX, p2_points<T, NX>()>::TT),
 k_p2(StepArgs<T> a) {
     using S = Team<NX, p2_points<T, NX>()>;
     constexpr int P = S::P, TT = S::TT, L = lds_len<NX>();
-    constexpr int NQ = (NX / 3 + 1 + TT - 1) / TT;   // >= ceil(nkxh / TT)
+    constexpr int NT = kTile * TT;
+    constexpr int NKMAX = NX / 3 + 1;                  // >= nkxh
+    constexpr int NQ = (NKMAX + TT - 1) / TT;          // kx per thread in the split
+    constexpr int NS = (4 * NKMAX + NT - 1) / NT;      // staged elements per thread per field
+    constexpr int KP = p2_stage_stride<NX>();
     __shared__ cplx<T> sm[kTile * L];
+    __shared__ double red[32];
     const Geo& g = a.g;
     const int team = threadIdx.x / TT, tid = threadIdx.x - team * TT;
     const int tile = blockIdx.x;
@@ -177,54 +190,98 @@ k_p2(StepArgs<T> a) {
     cplx<T>* lds = sm + team * L;
     const TeamSync<S::TT> sync{};
 
-    // x-inverse of field pair PP (0: velocities, 1: omega derivatives, 2: rho
-    // derivatives) for this team's column: Z(kx) = A + iB for kx >= 0 and
-    // conj(A) + i conj(B) at nx - kx, assembled branch-free from the LDS tile.
-    auto pair_ifft = [&](auto pc, cplx<T>* v) {
-        constexpr int PP = decltype(pc)::value;
-        const cplx<T>* GA = a.G + (long long)(2 * PP) * g.I + (long long)tile * nk * 4;
-        const cplx<T>* GB = GA + g.I;
-        for (int idx = threadIdx.x; idx < nk * 4; idx += blockDim.x) {
-            sm[idx] = GA[idx];
-            sm[nk * 4 + idx] = GB[idx];
+    // CFL maxima (add_v0 + pl_max_search): max|vx + sigma*yy[y]|, max|vy|, one
+    // partial pair per workgroup (P3 reduces the array; no atomics)
+    const double sy = __dmul_rn(a.ph.sigma, (double)y * g.dy);
+    double m0 = 0.0, m1 = 0.0;
+
+    // Inverse x-transforms of G field pairs FA + i FB for this team's column.
+    // Nonlinear: (vx, dw/dx), (vy, dw/dy), (vx, dr/dx), (vy, dr/dy), so each
+    // velocity meets the derivative it multiplies and only the two brackets
+    // stay live between transforms; linear: (vx, vy) for the CFL maxima only.
+    // The fields of G: 0 vx = X(i kys phi), 1 vy = X(-i kx phi), 2 X(i kx w),
+    // 3 X(i kys w), 4 X(i kx r), 5 X(i kys r).  The loop is not unrolled and
+    // its lane index is made opaque per pass, so addresses are recomputed
+    // rather than held in registers across transforms.
+    T bw[P], br[P];
+    const int npass = a.ph.linear ? 1 : 4;
+#pragma unroll 1
+    for (int pass = 0; pass < npass; ++pass) {
+        int lane = threadIdx.x;
+        asm volatile("" : "+v"(lane));
+        const int tl = lane - team * TT;
+        const int fa = a.ph.linear ? 0 : (pass & 1);
+        const int fb = a.ph.linear ? 1 : 2 + pass;
+        const cplx<T>* GA = a.G + (long long)fa * g.I + (long long)tile * nk * 4;
+        const cplx<T>* GB = a.G + (long long)fb * g.I + (long long)tile * nk * 4;
+        // stage the two tiles as [field][c][kx], all loads first
+        cplx<T> ra[NS], rb[NS];
+#pragma unroll
+        for (int m = 0; m < NS; ++m) {
+            const int idx = lane + m * NT;
+            const int id = idx < nk * 4 ? idx : 0;
+            ra[m] = GA[id];
+            rb[m] = GB[id];
+        }
+        teams_join<TT>();
+#pragma unroll
+        for (int m = 0; m < NS; ++m) {
+            const int idx = lane + m * NT;
+            if (idx < nk * 4) {
+                sm[(idx & 3) * KP + (idx >> 2)] = ra[m];
+                sm[(4 + (idx & 3)) * KP + (idx >> 2)] = rb[m];
+            }
         }
         __syncthreads();
+        // Z(kx) = A + iB for kx >= 0, conj(A) + i conj(B) at nx - kx,
+        // branch-free; real parts only at kx = 0
+        const cplx<T>* ca = sm + team * KP;
+        const cplx<T>* cb = sm + (4 + team) * KP;
+        cplx<T> v[P];
 #pragma unroll
         for (int m = 0; m < P; ++m) {
-            const int e = tid + m * TT;
+            const int e = tl + m * TT;
             const bool pos = e < nk;
             const bool keep = pos || e > NX - nk;
             const int kk = keep ? (pos ? e : NX - e) : 0;
-            const cplx<T> A = sm[kk * 4 + team], B = sm[nk * 4 + kk * 4 + team];
+            const cplx<T> A = ca[kk], B = cb[kk];
             const cplx<T> z = pos ? cplx<T>{A.x - B.y, A.y + B.x} : cplx<T>{A.x + B.y, B.x - A.y};
             v[m] = keep ? z : czero<T>();
         }
-        if (tid == 0) v[0] = {sm[team].x, sm[nk * 4 + team].x};   // kx = 0: real parts only
+        if (tl == 0) v[0] = {ca[0].x, cb[0].x};
         __syncthreads();
-        TeamFFT<T, NX, P, S::RMAX, +1>::run(v, lds, a.twx, tid, sync);
-    };
+        TeamFFT<T, NX, P, S::RMAX, +1>::run(v, lds, a.twx, tl, sync);
 
-    T vx[P], vy[P];
-    {
-        cplx<T> v[P];
-        pair_ifft(std::integral_constant<int, 0>{}, v);
+        if (a.ph.linear) {
 #pragma unroll
-        for (int m = 0; m < P; ++m) { vx[m] = v[m].x; vy[m] = v[m].y; }
+            for (int m = 0; m < P; ++m) {
+                m0 = fmax(m0, fabs(__dadd_rn((double)v[m].x, sy)));
+                m1 = fmax(m1, fabs((double)v[m].y));
+            }
+        } else if (pass == 0) {
+#pragma unroll
+            for (int m = 0; m < P; ++m) {
+                m0 = fmax(m0, fabs(__dadd_rn((double)v[m].x, sy)));
+                bw[m] = v[m].x * v[m].y;
+            }
+        } else if (pass == 1) {
+#pragma unroll
+            for (int m = 0; m < P; ++m) {
+                m1 = fmax(m1, fabs((double)v[m].x));
+                bw[m] += v[m].x * v[m].y;
+            }
+        } else if (pass == 2) {
+#pragma unroll
+            for (int m = 0; m < P; ++m) br[m] = v[m].x * v[m].y;
+        } else {
+#pragma unroll
+            for (int m = 0; m < P; ++m) br[m] += v[m].x * v[m].y;
+        }
     }
 
-    // CFL maxima (add_v0 + pl_max_search): max|vx + sigma*yy[y]|, max|vy|
+    m0 = wave_max(m0);
+    m1 = wave_max(m1);
     {
-        const double sy = __dmul_rn(a.ph.sigma, (double)y * g.dy);
-        double m0 = 0.0, m1 = 0.0;
-#pragma unroll
-        for (int m = 0; m < P; ++m) {
-            m0 = fmax(m0, fabs(__dadd_rn((double)vx[m], sy)));
-            m1 = fmax(m1, fabs((double)vy[m]));
-        }
-        // one partial pair per workgroup, no atomics (P3 reduces the array)
-        __shared__ double red[32];
-        m0 = wave_max(m0);
-        m1 = wave_max(m1);
         const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
         if ((threadIdx.x & 63) == 0) { red[w] = m0; red[16 + w] = m1; }
         __syncthreads();
@@ -236,18 +293,10 @@ k_p2(StepArgs<T> a) {
     }
     if (a.ph.linear) return;
 
-    // brackets B = vx*df/dx + vy*df/dy (calc_poisson_bracket), negated (negative)
-    T bw[P];
-    {
-        cplx<T> v[P];
-        pair_ifft(std::integral_constant<int, 1>{}, v);
-#pragma unroll
-        for (int m = 0; m < P; ++m) bw[m] = vx[m] * v[m].x + vy[m] * v[m].y;
-    }
+    // the negated brackets (negative) as z = -B_omega - i B_rho
     cplx<T> v[P];
-    pair_ifft(std::integral_constant<int, 2>{}, v);
 #pragma unroll
-    for (int m = 0; m < P; ++m) v[m] = {-bw[m], -(vx[m] * v[m].x + vy[m] * v[m].y)};
+    for (int m = 0; m < P; ++m) v[m] = {-bw[m], -br[m]};
 
     // x-forward of z = -B_omega - i B_rho, then split into the two spectra
     TeamFFT<T, NX, P, S::RMAX, -1>::run(v, lds, a.twx, tid, sync);
@@ -265,20 +314,25 @@ k_p2(StepArgs<T> a) {
         }
     }
     __syncthreads();
+    // stage as [field][c][kx], then write H as [t][kx][c] tiles
 #pragma unroll
     for (int i = 0; i < NQ; ++i) {
         const int kx = tid + i * TT;
         if (kx < nk) {
-            sm[kx * 4 + team] = hw[i];
-            sm[nk * 4 + kx * 4 + team] = hr[i];
+            sm[team * KP + kx] = hw[i];
+            sm[(4 + team) * KP + kx] = hr[i];
         }
     }
     __syncthreads();
     cplx<T>* Hw = a.H + (long long)tile * nk * 4;
     cplx<T>* Hr = Hw + g.I;
-    for (int idx = threadIdx.x; idx < nk * 4; idx += blockDim.x) {
-        Hw[idx] = sm[idx];
-        Hr[idx] = sm[nk * 4 + idx];
+#pragma unroll
+    for (int m = 0; m < NS; ++m) {
+        const int idx = threadIdx.x + m * NT;
+        if (idx < nk * 4) {
+            Hw[idx] = sm[(idx & 3) * KP + (idx >> 2)];
+            Hr[idx] = sm[(4 + (idx & 3)) * KP + (idx >> 2)];
+        }
     }
 }
 
@@ -322,7 +376,7 @@ __device__ __forceinline__ cplx<T> ab_bdf_update(int ist, bool plain, double dt,
     if (!plain) {                                              // eulerb / bdf2
         const double cd = ist == 0 ? dt : __dmul_rn(2.0 / 3.0, dt);
         const T den = (T)__dadd_rn(1.0, __dmul_rn(__dmul_rn(cd, diff), k2));
-        r = {r.x / den, r.y / den};
+        r = cdivr(r, den);
     }
     return r;
 }
@@ -389,12 +443,19 @@ __global__ void __launch_bounds__(kTile * TeamShape<NY>::TT, (waves_per_eu<T, NY
 k_p3(StepArgs<T> a) {
     using S = TeamShape<NY>;
     constexpr int P = S::P, TT = S::TT, L = lds_len<NY>();
+    constexpr int NT = kTile * TT;
+    constexpr int LP = stage_stride<NY>();
+    constexpr int J = (NY / 3 + 1 + TT - 1) / TT;   // row points per thread (nky <= NY/3 + 1)
+    constexpr int JB = sizeof(T) == 4 ? 3 : 2;      // points whose loads are in flight together
     __shared__ cplx<T> sm[kTile * L];
+    __shared__ double red[32];
     const Geo& g = a.g;
     const Phys& ph = a.ph;
     const int team = threadIdx.x / TT, tid = threadIdx.x - team * TT;
-    const int f = blockIdx.x / g.ntile_x;   // 0: omega, 1: rho
-    const int tile = blockIdx.x - f * g.ntile_x;
+    // the omega and rho workgroups of a tile read each other's rows: keep them on one XCD
+    const int q = xcd_item(blockIdx.x, gridDim.x);
+    const int tile = q >> 1;
+    const int f = q & 1;   // 0: omega, 1: rho
     const int row = tile * kTile + team;
     const bool act = row < g.nkxh;
     const int rn = row == 0 ? 0 : g.nkx - row;
@@ -404,14 +465,6 @@ k_p3(StepArgs<T> a) {
     cplx<T>* lds = sm + team * L;
     const TeamSync<S::TT> sync{};
 
-    double dt;
-    int which;
-    __shared__ double red[32];
-    double cvx, cvy;
-    cfl_maxima(a.cflp, g.ntile_y, red, &cvx, &cvy);
-    const int h = cfl_delt(c, g, ph, cvx, cvy, &dt, &which);
-    if (blockIdx.x == 0) p3_bookkeeping(a, dt, h, which, cvx, cvy);
-
     const int s0 = slot_a0(n), s1 = slot_a1(n), s2 = slot_a2(n);
     const int* jbase = a.jump + (c->epoch & 1) * g.nkx;
     const int* ja1 = c->slot_epoch_a[s1] != c->epoch ? jbase : nullptr;
@@ -420,72 +473,139 @@ k_p3(StepArgs<T> a) {
     const int* jd2 = c->slot_epoch_d[s2] != c->epoch ? jbase : nullptr;
     const int ist = n == 0 ? 0 : (n == 1 ? 1 : 2);
     const T scale = (T)g.scale;
+    double dt = 0.0, cvx = 0.0, cvy = 0.0;
+    int which = -1;
 
-    {
+    cplx<T>* A0 = f ? a.a_rho[s0] : a.a_omg[s0];
+    const cplx<T>* A1 = f ? a.a_rho[s1] : a.a_omg[s1];
+    const cplx<T>* A2 = f ? a.a_rho[s2] : a.a_omg[s2];
+    cplx<T>* D0 = f ? a.d_rho[s0] : a.d_omg[s0];
+    const cplx<T>* D1 = f ? a.d_rho[s1] : a.d_omg[s1];
+    const cplx<T>* D2 = f ? a.d_rho[s2] : a.d_omg[s2];
+    const cplx<T>* X1 = f ? a.a_omg[s1] : a.a_rho[s1];   // the other field at step n
+    const double diff = f ? ph.kappa : ph.nu;
+    const bool plain = diff < ph.eps;
+
+    // The update streams its operands in batches of JB points of one row (side
+    // 0: kx, side 1: -kx); the loads of batch b+1 are issued before batch b is
+    // computed, and those of batch 0 before the H transform, so the transform
+    // runs while the first operands arrive.
+    constexpr int NBS = (J + JB - 1) / JB;   // batches per side
+    constexpr int NB = 2 * NBS;
+    struct Batch {
+        cplx<T> x1[JB], a1[JB], a2[JB], d1[JB], d2[JB];
+        double ky[JB];
+    };
+    // kx and the shift of the two rows, loaded before anything is in flight
+    const int rc = act ? row : 0;
+    const double kx_r[2] = {a.kx[rc], a.kx[rc == 0 ? 0 : g.nkx - rc]};
+    const double s_r[2] = {a.s_in[rc], a.s_in[rc == 0 ? 0 : g.nkx - rc]};
+    // Mode (r, ik) of a ring buffer: through the remap view only in the steps
+    // right after a remap (VIEW), a plain load otherwise.
+    auto ld = [&](auto view, const cplx<T>* buf, int r, int ik, const int* jmp) -> cplx<T> {
+        if constexpr (decltype(view)::value) return view_load(buf, r, ik, g, jmp);
+        else return buf[(long long)r * g.nky + ik];
+    };
+    auto load_batch = [&](auto view, auto bc, Batch& B) {
+        constexpr int b = decltype(bc)::value, side = b / NBS, j0 = (b % NBS) * JB;
+        if (!act || (side == 1 && row == 0)) return;
+        const int r = side ? rn : row;
+#pragma unroll
+        for (int u = 0; u < JB; ++u) {
+            const int iky = tid + (j0 + u) * TT;
+            const int ik = (j0 + u < J && iky < g.nky) ? iky : 0;   // in bounds for idle lanes
+            // unconditional: the first steps load history slots they then ignore
+            B.x1[u] = ld(view, X1, r, ik, ja1);
+            B.a1[u] = ld(view, A1, r, ik, ja1);
+            B.a2[u] = ld(view, A2, r, ik, ja2);
+            B.d1[u] = ld(view, D1, r, ik, jd1);
+            B.d2[u] = ld(view, D2, r, ik, jd2);
+            B.ky[u] = a.ky[ik];
+        }
+    };
+    auto update_batch = [&](auto bc, const Batch& B) {
+        constexpr int b = decltype(bc)::value, side = b / NBS, j0 = (b % NBS) * JB;
+        if (!act || (side == 1 && row == 0)) return;
+        const int r = side ? rn : row;
+        const double kxv = kx_r[side];
+        const double sr = s_r[side];
+        // add_ddxrho: g/rho0 * kx;  add_ddxphi: rho0' * kx
+        const T cc = (T)__dmul_rn(f == 0 ? ph.g / ph.rho0 : ph.rho0_prime, kxv);
+#pragma unroll
+        for (int u = 0; u < JB; ++u) {
+            const int iky = tid + (j0 + u) * TT;
+            if (j0 + u >= J || iky >= g.nky) continue;
+            cplx<T> nh = czero<T>();
+            if (!ph.linear) {
+                const cplx<T> F = side ? cconj(lds[(NY - iky) & (NY - 1)]) : lds[iky];
+                nh = {F.x * scale, F.y * scale};
+            }
+            const double k2 = k2_of(kxv, __dadd_rn(B.ky[u], sr));
+            cplx<T> d0;
+            if (f == 0) {
+                d0 = {nh.x + cc * B.x1[u].y, nh.y - cc * B.x1[u].x};
+            } else {   // phi = omega/kperp2 (zero at the origin)
+                const cplx<T> phi = (r == 0 && iky == 0) ? czero<T>() : cdivr(B.x1[u], (T)k2);
+                d0 = {nh.x - cc * phi.y, nh.y + cc * phi.x};
+            }
+            const long long at = (long long)r * g.nky + iky;
+            A0[at] = ab_bdf_update<T>(ist, plain, dt, diff, k2, B.a1[u], B.a2[u], d0, B.d1[u], B.d2[u]);
+            D0[at] = d0;
+        }
+    };
+
+    auto run = [&](auto view) {
+        Batch bt[2];
+
+        // the step size after the CFL halvings, and (workgroup 0) the bookkeeping
+        cfl_maxima(a.cflp, g.ntile_y, red, &cvx, &cvy);
+        const int h = cfl_delt(c, g, ph, cvx, cvy, &dt, &which);
+        if (blockIdx.x == 0) p3_bookkeeping(a, dt, h, which, cvx, cvy);
+
         if (!ph.linear) {
+            // twiddles and the H tile of this field, then [team][y] in LDS
+            using FFT = TeamFFT<T, NY, P, S::RMAX, -1>;
+            typename FFT::Twiddles ts;
+            ts.load(a.twy, tid);
             const cplx<T>* Hf = a.H + (long long)f * g.I;
-            for (int idx = threadIdx.x; idx < NY * 4; idx += blockDim.x) {
+            cplx<T> hv[P];
+#pragma unroll
+            for (int m = 0; m < P; ++m) {   // NY*4 tile elements over NT threads
+                const int idx = threadIdx.x + m * NT;
                 const int t = idx >> 4, w = idx & 15;
-                sm[idx] = (w >> 2) < nrow_tile ? Hf[((long long)t * g.nkxh + tile * kTile) * 4 + w] : czero<T>();
+                hv[m] = (w >> 2) < nrow_tile ? Hf[((long long)t * g.nkxh + tile * kTile) * 4 + w] : czero<T>();
+            }
+#pragma unroll
+            for (int m = 0; m < P; ++m) {
+                const int idx = threadIdx.x + m * NT;
+                const int t = idx >> 4, w = idx & 15;
+                sm[(w >> 2) * LP + 4 * t + (w & 3)] = hv[m];
             }
             __syncthreads();
             cplx<T> v[P];
 #pragma unroll
-            for (int m = 0; m < P; ++m) v[m] = sm[stage_index(tid, team) + 4 * TT * m];
+            for (int m = 0; m < P; ++m) v[m] = sm[team * LP + tid + m * TT];
             __syncthreads();
-            TeamFFT<T, NY, P, S::RMAX, -1>::run(v, lds, a.twy, tid, sync);
+            // the first operands travel during the transform (loads complete in
+            // order, so they are issued after the H tile has been consumed)
+            load_batch(view, std::integral_constant<int, 0>{}, bt[0]);
+            FFT::run(v, lds, ts, tid, sync);
 #pragma unroll
             for (int m = 0; m < P; ++m) lds[tid + m * TT] = v[m];
             __syncthreads();
+        } else {
+            load_batch(view, std::integral_constant<int, 0>{}, bt[0]);
         }
-        if (act) {
-            cplx<T>* A0 = f ? a.a_rho[s0] : a.a_omg[s0];
-            const cplx<T>* A1 = f ? a.a_rho[s1] : a.a_omg[s1];
-            const cplx<T>* A2 = f ? a.a_rho[s2] : a.a_omg[s2];
-            cplx<T>* D0 = f ? a.d_rho[s0] : a.d_omg[s0];
-            const cplx<T>* D1 = f ? a.d_rho[s1] : a.d_omg[s1];
-            const cplx<T>* D2 = f ? a.d_rho[s2] : a.d_omg[s2];
-            const cplx<T>* X1 = f ? a.a_omg[s1] : a.a_rho[s1];   // the other field at step n
-            const double diff = f ? ph.kappa : ph.nu;
-            const bool plain = diff < ph.eps;
-            for (int iky = tid; iky < g.nky; iky += TT) {
-#pragma unroll
-                for (int side = 0; side < 2; ++side) {
-                    if (side == 1 && row == 0) break;
-                    const int r = side ? rn : row;
-                    cplx<T> nh = czero<T>();
-                    if (!ph.linear) {
-                        const cplx<T> F = side ? cconj(lds[(NY - iky) & (NY - 1)]) : lds[iky];
-                        nh = {F.x * scale, F.y * scale};
-                    }
-                    const double kxv = a.kx[r];
-                    const double k2 = k2_of(kxv, kys_of(a.ky, a.s_in, r, iky));
-                    const cplx<T> x1 = view_load(X1, r, iky, g, ja1);
-                    cplx<T> d0;
-                    if (f == 0) {                               // add_ddxrho
-                        const T cc = (T)__dmul_rn(ph.g / ph.rho0, kxv);
-                        d0 = {nh.x + cc * x1.y, nh.y - cc * x1.x};
-                    } else {                                    // add_ddxphi with phi = omega/kperp2
-                        cplx<T> phi = czero<T>();
-                        if (!(r == 0 && iky == 0)) {
-                            const T k2t = (T)k2;
-                            phi = {x1.x / k2t, x1.y / k2t};
-                        }
-                        const T cc = (T)__dmul_rn(ph.rho0_prime, kxv);
-                        d0 = {nh.x - cc * phi.y, nh.y + cc * phi.x};
-                    }
-                    const cplx<T> a1 = view_load(A1, r, iky, g, ja1);
-                    const cplx<T> a2 = ist >= 1 ? view_load(A2, r, iky, g, ja2) : czero<T>();
-                    const cplx<T> d1 = ist >= 1 ? view_load(D1, r, iky, g, jd1) : czero<T>();
-                    const cplx<T> d2 = ist >= 2 ? view_load(D2, r, iky, g, jd2) : czero<T>();
-                    const long long at = (long long)r * g.nky + iky;
-                    A0[at] = ab_bdf_update<T>(ist, plain, dt, diff, k2, a1, a2, d0, d1, d2);
-                    D0[at] = d0;
-                }
-            }
-        }
-        __syncthreads();
-    }
+
+        static_for<0, NB>([&](auto bc) {
+            constexpr int b = decltype(bc)::value;
+            if constexpr (b + 1 < NB) load_batch(view, std::integral_constant<int, b + 1>{}, bt[(b + 1) & 1]);
+            asm volatile("" ::: "memory");   // keep those loads ahead of this batch's stores
+            update_batch(bc, bt[b & 1]);
+        });
+    };
+    if (ja1 != nullptr || ja2 != nullptr || jd1 != nullptr || jd2 != nullptr) run(std::true_type{});
+    else run(std::false_type{});
 }
 
 }  // namespace shrt

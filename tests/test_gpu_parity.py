@@ -28,7 +28,8 @@ def pair(n, prec, ny=None, graph_steps=0, **kw):
 
 
 @pytest.mark.parametrize("prec", [8, 4])
-@pytest.mark.parametrize("nx,ny", [(32, 32), (64, 64), (128, 128), (64, 128), (256, 256)])
+@pytest.mark.parametrize("nx,ny", [(32, 32), (64, 64), (128, 128), (64, 128), (256, 256), (512, 512),
+                                   (1024, 1024)])
 def test_xtok_ktox(gpu, prec, nx, ny):
     print(f"xtok/ktox nx={nx} ny={ny} prec={prec}", flush=True)
     o, g = pair(nx, prec, ny=ny)
@@ -111,6 +112,25 @@ def test_multistep_seeded(gpu, prec, n, steps):
     so, sg = o.state(), g.state()
     assert sg["istep"] == so["istep"]
     assert abs(sg["time"] - so["time"]) < 1e-12
+
+
+@pytest.mark.parametrize("prec", [8, 4])
+@pytest.mark.parametrize("nx,ny,steps", [(64, 128, 60), (512, 512, 8), (1024, 1024, 4), (2048, 2048, 3)])
+def test_multistep_sizes(gpu, prec, nx, ny, steps):
+    """Non-square grids and the large transform lengths (one-wave teams at 512
+    and 1024, the headline length 2048) against the oracle."""
+    print(f"multistep nx={nx} ny={ny} prec={prec} steps={steps}", flush=True)
+    o, g = pair(nx, prec, ny=ny, sigma=1.0)
+    ic = dict(seed=4242, kmin=2.0, kmax=24.0, amp=1e-3, expo=-1.0)
+    o.set_spectral("rho", oracle.seeded_spectrum(nx, ny, 2 * np.pi, 2 * np.pi, **ic))
+    g.initialize_seeded(**ic)
+    o.step(steps)
+    g.time_advance(steps)
+    for f, ring in ((shrt.OMEGA, "aomg"), (shrt.RHO, "arho"), (shrt.PHI, "aphi")):
+        err = rel_l2(g.spectral(f), o.buffer(ring))
+        print(f"  field {f} rel err {err:.3e}")
+        assert err < TOLN[prec]
+    assert g.state()["istep"] == o.state()["istep"]
 
 
 @pytest.mark.parametrize("prec", [8, 4])
