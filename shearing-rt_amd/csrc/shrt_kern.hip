@@ -65,9 +65,8 @@ hipError_t Launch<R>::p1(const StepArgs<R>& a, int grid, hipStream_t s) {
     (void)grid;
     bool ok = dispatch(a.g.ny, [&](auto nc) {
         constexpr int N = decltype(nc)::value;
-        constexpr int OPW = p1_outputs_per_wg<R, N>();
-        const int chunks = ((a.ph.linear ? 2 : 6) + OPW - 1) / OPW;
-        hipLaunchKernelGGL((k_p1<R, N>), dim3(a.g.ntile_x * chunks), dim3(kTile * TeamShape<N>::TT), 0, s, a);
+        const P1Grid pg(a.g.nkxh, a.ph.linear);
+        hipLaunchKernelGGL((k_p1<R, N>), dim3(pg.grid), dim3(kTile * TeamShape<N>::TT), 0, s, a);
     });
     return ok ? hipGetLastError() : unsupported();
 }

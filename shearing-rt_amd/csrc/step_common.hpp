@@ -34,10 +34,6 @@ constexpr int waves_per_eu() {
 
 // Points per thread of each step kernel.
 template <typename T, int N> constexpr int p1_points() { return TeamShape<N>::P; }
-// Outputs (of the six y-transformed fields) per P1 workgroup: 1, 2, 3 or 6.
-// One output per workgroup measured fastest at 2048^2 FP32 (45 us against 51 us
-// for two and 47 us for three).
-template <typename T, int N> constexpr int p1_outputs_per_wg() { return 1; }
 template <typename T, int N> constexpr int p2_points() { return TeamShape<N>::P; }
 template <typename T, int N> constexpr int p3_points() { return TeamShape<N>::P; }
 

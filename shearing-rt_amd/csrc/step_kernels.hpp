@@ -63,33 +63,53 @@ __device__ __forceinline__ cplx<T> p1_mult(const ModeK<T>& k, bool origin, cplx<
     }
 }
 
-// One output OI (0..5) of P1 for a tile of kTile row pairs (kx, -kx): group
-// GRP = OI/2 (0: phi -> vx, vy; 1: omega; 2: rho), output O = OI%2.
+// G stores the P1 rows by position: position p holds row p + 1 for p < nkxh - 1
+// and row 0 (kx = 0) at p = nkxh - 1.  Full tiles of kTile positions then hold
+// rows 1..nkxh-1 except for a remainder that shares a few workgroups with row
+// 0, whose kx-multiplied outputs vanish: at 2048^2 the grid is exactly 1024
+// workgroups instead of 1026 (four times the 256 CUs).
+__host__ __device__ inline int p1_row_of(int p, int nkxh) { return p == nkxh - 1 ? 0 : p + 1; }
+__host__ __device__ inline int g_pos_of(int kx, int nkxh) { return kx == 0 ? nkxh - 1 : kx - 1; }
+
+struct P1Grid {
+    int nout;      // outputs per row: 6, or 2 in the linear mode
+    int nfull;     // full tiles (rows 1 .. 4*nfull)
+    int rem;       // rows left over, row 0 included (1..kTile)
+    int nspec;     // transforms of the leftover rows (row 0 needs only vx, d/dy omega, d/dy rho)
+    int nspec_wg;  // workgroups that run them, one transform per team
+    int grid;
+    __host__ __device__ P1Grid(int nkxh, int linear) {
+        nout = linear ? 2 : 6;
+        nfull = (nkxh - 1) / kTile;
+        rem = nkxh - nfull * kTile;
+        nspec = nout * (rem - 1) + (linear ? 1 : 3);
+        nspec_wg = (nspec + kTile - 1) / kTile;
+        grid = nfull * nout + nspec_wg;
+    }
+};
+
+// The outputs of row 0 that are transformed (the others are zero).
+__host__ __device__ inline int p1_row0_output(int j) { return j == 0 ? 0 : (j == 1 ? 3 : 5); }
+
+// y-spectrum of output OI for the row pair (row, -row) in natural order,
+// branch-free: element e takes row kx at iky = e (e < nky) or the conjugate of
+// row -kx at iky = NY - e (e > NY - nky); padding lanes load an in-bounds
+// element and are zeroed.  Group GRP = OI/2 (0: phi -> vx, vy; 1: omega;
+// 2: rho), output O = OI%2.
 template <int OI, typename T, int NY>
-__device__ __forceinline__ void p1_output(const StepArgs<T>& a, cplx<T>* sm, int team, int tid, int tile) {
+__device__ __forceinline__ void p1_spectrum(const StepArgs<T>& a, int row, bool act, int tid, cplx<T>* v) {
     constexpr int GRP = OI / 2, O = OI % 2;
     using S = TeamShape<NY>;
-    constexpr int P = S::P, TT = S::TT, L = lds_len<NY>();
+    constexpr int P = S::P, TT = S::TT;
     const Geo& g = a.g;
-    const int row = tile * kTile + team;
-    const bool act = row < g.nkxh;
     const int rowc = act ? row : 0;
     const int rn = rowc == 0 ? 0 : g.nkx - rowc;
     const StepCtl* c = a.ctl_in;
     const int s1 = slot_a1(c->istep);
     const cplx<T>* src = GRP == 2 ? a.a_rho[s1] : a.a_omg[s1];
     const int* jmp = (c->slot_epoch_a[s1] != c->epoch) ? a.jump + (c->epoch & 1) * g.nkx : nullptr;
-    cplx<T>* lds = sm + team * L;
-    const TeamSync<S::TT> sync{};
     const double kxp = a.kx[rowc], kxn = a.kx[rn];
     const double sp = a.s_in[rowc], sn = a.s_in[rn];
-    const int nrow_tile = min(kTile, g.nkxh - tile * kTile);
-
-    // y-spectrum of the row pair in natural order, branch-free: element e
-    // takes row kx at iky = e (e < nky) or the conjugate of row -kx at
-    // iky = NY - e (e > NY - nky); padding lanes load an in-bounds element
-    // and are zeroed.
-    cplx<T> v[P];
 #pragma unroll
     for (int m = 0; m < P; ++m) {
         const int e = tid + m * TT;
@@ -108,62 +128,109 @@ __device__ __forceinline__ void p1_output(const StepArgs<T>& a, cplx<T>* sm, int
         const cplx<T> vn = p1_mult<GRP, O, T>(k, rn == 0, view_load(src, rn, 0, g, jmp));
         v[0] = {(v[0].x + vn.x) * T(0.5), (v[0].y - vn.y) * T(0.5)};
     }
-    TeamFFT<T, NY, P, S::RMAX, +1>::run(v, lds, a.twy, tid, sync);
-    // stage as [team][y], then write G tile by tile: 4x4 blocks [t][team][c]
-    // (y = 4t + c) are contiguous runs of G
-    constexpr int LP = stage_stride<NY>();
-    teams_join<TT>();
-#pragma unroll
-    for (int m = 0; m < P; ++m) sm[team * LP + tid + m * TT] = v[m];
-    __syncthreads();
-    cplx<T>* Gf = a.G + (long long)OI * g.I;
-#pragma unroll
-    for (int m = 0; m < P; ++m) {   // NY*4 staged elements over kTile*TT threads
-        const int idx = threadIdx.x + m * kTile * TT;
-        const int t = idx >> 4, w = idx & 15;
-        if ((w >> 2) < nrow_tile)
-            Gf[((long long)t * g.nkxh + tile * kTile) * 4 + w] = sm[(w >> 2) * LP + 4 * t + (w & 3)];
-    }
-    __syncthreads();
+}
 
-    // aphi (shear.cu:367-374) for the rows of this tile, when requested
-    if constexpr (OI == 0) {
-        if (a.store_phi && act) {
-            for (int iky = tid; iky < g.nky; iky += TT) {
+// aphi = omega/kperp2_shear (neg_lapinv_shear, shear.cu:367-374) for the row
+// pair (row, -row), by one team.
+template <typename T, int NY>
+__device__ __forceinline__ void p1_store_phi(const StepArgs<T>& a, int row, int tid) {
+    constexpr int TT = TeamShape<NY>::TT;
+    const Geo& g = a.g;
+    const StepCtl* c = a.ctl_in;
+    const int s1 = slot_a1(c->istep);
+    const cplx<T>* src = a.a_omg[s1];
+    const int* jmp = (c->slot_epoch_a[s1] != c->epoch) ? a.jump + (c->epoch & 1) * g.nkx : nullptr;
+    const int rn = row == 0 ? 0 : g.nkx - row;
+    for (int iky = tid; iky < g.nky; iky += TT) {
 #pragma unroll
-                for (int side = 0; side < 2; ++side) {
-                    if (side == 1 && row == 0) break;
-                    const int r = side ? rn : row;
-                    const ModeK<T> k = mode_k<T>(a.ky, side ? sn : sp, side ? kxn : kxp, iky);
-                    const cplx<T> F = view_load(src, r, iky, g, jmp);
-                    a.aphi[(long long)r * g.nky + iky] =
-                        (r == 0 && iky == 0) ? czero<T>() : cdivr(F, k.k2);
-                }
-            }
+        for (int side = 0; side < 2; ++side) {
+            if (side == 1 && row == 0) break;
+            const int r = side ? rn : row;
+            const ModeK<T> k = mode_k<T>(a.ky, a.s_in[r], a.kx[r], iky);
+            const cplx<T> F = view_load(src, r, iky, g, jmp);
+            a.aphi[(long long)r * g.nky + iky] = (r == 0 && iky == 0) ? czero<T>() : cdivr(F, k.k2);
         }
     }
 }
 
-// Workgroups of P1: ntile_x row tiles times 6/OPW output chunks; workgroup
-// (tile, chunk) produces outputs chunk*OPW .. chunk*OPW + OPW - 1 (only 0 and 1
-// in the linear mode).
+// Workgroups of P1.  Items q < nfull*nout: full tile q / nout (positions
+// 4t..4t+3, rows 4t+1..4t+4), output q % nout, one row per team, written to G
+// through [team][y] staging in 4x4 blocks [t][position][c] (y = 4t + c).
+// Items beyond: the leftover rows, one (row, output) transform per team,
+// written by the team directly; the first of them also writes the vanishing
+// outputs of row 0.
 template <typename T, int NY>
 __global__ void __launch_bounds__(kTile * TeamShape<NY>::TT, (waves_per_eu<T, NY>()))
 k_p1(StepArgs<T> a) {
     using S = TeamShape<NY>;
-    constexpr int OPW = p1_outputs_per_wg<T, NY>();
-    __shared__ cplx<T> sm[kTile * lds_len<NY>()];
-    const int team = threadIdx.x / S::TT, tid = threadIdx.x - team * S::TT;
-    const int last = a.ph.linear ? 2 : 6;
-    const int chunks = (last + OPW - 1) / OPW;
-    // the chunks of one tile read the same rows: keep them on one XCD
+    constexpr int P = S::P, TT = S::TT, L = lds_len<NY>();
+    constexpr int LP = stage_stride<NY>();
+    __shared__ cplx<T> sm[kTile * L];
+    const Geo& g = a.g;
+    const int team = threadIdx.x / TT, tid = threadIdx.x - team * TT;
+    const P1Grid pg(g.nkxh, a.ph.linear);
+    // the outputs of one tile read the same rows: keep them on one XCD
     const int q = xcd_item(blockIdx.x, gridDim.x);
-    const int tile = q / chunks;
-    const int first = (q - tile * chunks) * OPW;
+    cplx<T>* lds = sm + team * L;
+    const TeamSync<S::TT> sync{};
+    cplx<T> v[P];
+
+    if (q < pg.nfull * pg.nout) {
+        const int tile = q / pg.nout;
+        const int oi = q - tile * pg.nout;
+        const int row = tile * kTile + team + 1;
+        static_for<0, 6>([&](auto oc) {
+            constexpr int OI = decltype(oc)::value;
+            if (OI == oi) p1_spectrum<OI, T, NY>(a, row, true, tid, v);
+        });
+        TeamFFT<T, NY, P, S::RMAX, +1>::run(v, lds, a.twy, tid, sync);
+        teams_join<TT>();
+#pragma unroll
+        for (int m = 0; m < P; ++m) sm[team * LP + tid + m * TT] = v[m];
+        __syncthreads();
+        cplx<T>* Gf = a.G + (long long)oi * g.I;
+#pragma unroll
+        for (int m = 0; m < P; ++m) {   // NY*4 staged elements over kTile*TT threads
+            const int idx = threadIdx.x + m * kTile * TT;
+            const int t = idx >> 4, w = idx & 15;
+            Gf[((long long)t * g.nkxh + tile * kTile) * 4 + w] = sm[(w >> 2) * LP + 4 * t + (w & 3)];
+        }
+        if (oi == 0 && a.store_phi) p1_store_phi<T, NY>(a, row, tid);
+        return;
+    }
+
+    // leftover rows: item i of this team
+    const int j = q - pg.nfull * pg.nout;
+    const int i = j * kTile + team;
+    const bool act = i < pg.nspec;
+    const int nreg = pg.nout * (pg.rem - 1);
+    const int pos = i < nreg ? pg.nfull * kTile + i / pg.nout : g.nkxh - 1;
+    const int oi = i < nreg ? i % pg.nout : p1_row0_output(i - nreg);
+    const int row = p1_row_of(pos, g.nkxh);
     static_for<0, 6>([&](auto oc) {
         constexpr int OI = decltype(oc)::value;
-        if (OI >= first && OI < first + OPW && OI < last) p1_output<OI, T, NY>(a, sm, team, tid, tile);
+        if (OI == oi) p1_spectrum<OI, T, NY>(a, row, act, tid, v);
     });
+    TeamFFT<T, NY, P, S::RMAX, +1>::run(v, lds, a.twy, tid, sync);
+    if (act) {
+        cplx<T>* Gf = a.G + (long long)oi * g.I;
+#pragma unroll
+        for (int m = 0; m < P; ++m) {
+            const int y = tid + m * TT;
+            Gf[((long long)(y >> 2) * g.nkxh + pos) * 4 + (y & 3)] = v[m];
+        }
+        if (oi == 0 && a.store_phi) p1_store_phi<T, NY>(a, row, tid);
+    }
+    if (j == 0) {
+        // outputs of row 0 that carry a factor kx = 0
+        const int p0 = g.nkxh - 1;
+        const int nz = a.ph.linear ? 1 : 3;
+        for (int idx = threadIdx.x; idx < nz * NY; idx += blockDim.x) {
+            const int z = idx / NY, y = idx - z * NY;
+            const int o = z == 0 ? 1 : (z == 1 ? 2 : 4);
+            a.G[(long long)o * g.I + ((long long)(y >> 2) * g.nkxh + p0) * 4 + (y & 3)] = czero<T>();
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -244,11 +311,12 @@ k_p2(StepArgs<T> a) {
             const bool pos = e < nk;
             const bool keep = pos || e > NX - nk;
             const int kk = keep ? (pos ? e : NX - e) : 0;
-            const cplx<T> A = ca[kk], B = cb[kk];
+            const int gp = g_pos_of(kk, nk);   // G position of row kk
+            const cplx<T> A = ca[gp], B = cb[gp];
             const cplx<T> z = pos ? cplx<T>{A.x - B.y, A.y + B.x} : cplx<T>{A.x + B.y, B.x - A.y};
             v[m] = keep ? z : czero<T>();
         }
-        if (tl == 0) v[0] = {ca[0].x, cb[0].x};
+        if (tl == 0) v[0] = {ca[nk - 1].x, cb[nk - 1].x};   // kx = 0: real parts only
         __syncthreads();
         TeamFFT<T, NX, P, S::RMAX, +1>::run(v, lds, a.twx, tl, sync);
 
