@@ -132,14 +132,21 @@ shrt_status shrt_set_spectral(shrt_ctx* ctx, int field, const void* host);
 shrt_status shrt_get_buffer(shrt_ctx* ctx, int ring, int slot, void* host);
 shrt_status shrt_set_buffer(shrt_ctx* ctx, int ring, int slot, const void* host);
 
-/* Physical field (ktox, fft.cu:170-204) of omega, phi, rho or the velocities, host
- * buffer of nx*ny reals.  lab_frame must be 0 (the lab-frame transform
- * ktox_shear, shear.cu:467-478, is not implemented yet: SHRT_ERR_UNSUPPORTED). */
+/* Physical field of omega, phi, rho or the velocities, host buffer of nx*ny
+ * reals.  lab_frame = 0: ktox (fft.cu:170-204) in the sheared frame; lab_frame
+ * = 1: what output_fields writes (file_access.cu:250-259), ktox_shear
+ * (shear.cu:467-478) with the current ky_shift when sigma != 0, ktox otherwise.
+ * The velocities (those of the last CFL check, time_integral.cu:354) exist in
+ * the sheared frame only. */
 shrt_status shrt_get_physical(shrt_ctx* ctx, int pfield, int lab_frame, void* host);
 
 /* xtok / ktox (fft.h:17-25) on host buffers. */
 shrt_status shrt_xtok(shrt_ctx* ctx, const void* phys_in, void* spec_out);
 shrt_status shrt_ktox(shrt_ctx* ctx, const void* spec_in, void* phys_out);
+
+/* ktox_shear (shear.cu:467-478) on a host buffer with the context's current
+ * ky_shift: the lab-frame physical field of a spectral field. */
+shrt_status shrt_ktox_shear(shrt_ctx* ctx, const void* spec_in, void* phys_out);
 
 /* shearing_field (shear.cu:296-322): remap one spectral buffer with the given
  * jump[nkx] (ikx_indexed fold for jump > 0). */

@@ -62,6 +62,7 @@ def lib():
         L.orc_set_shift.argtypes = [vp, dp]
         L.orc_xtok.argtypes = [vp, dp, dp]
         L.orc_ktox.argtypes = [vp, dp, dp]
+        L.orc_ktox_shear.argtypes = [vp, dp, dp]
         L.orc_shearing_field.argtypes = [vp, ip, dp, dp]
         L.orc_poisson_bracket.argtypes = [vp, dp, dp]
         L.orc_check_cfl.argtypes = [vp]
@@ -187,6 +188,12 @@ class Oracle:
         a = np.ascontiguousarray(spec, dtype=np.complex128).reshape(-1)
         out = np.zeros(self.nx * self.ny)
         lib().orc_ktox(self.h, _dp(a.view(np.float64)), _dp(out))
+        return out.reshape(self.nx, self.ny)
+
+    def ktox_shear(self, spec):
+        a = np.ascontiguousarray(spec, dtype=np.complex128).reshape(-1)
+        out = np.zeros(self.nx * self.ny)
+        lib().orc_ktox_shear(self.h, _dp(a.view(np.float64)), _dp(out))
         return out.reshape(self.nx, self.ny)
 
     def shearing_field(self, jump, spec):

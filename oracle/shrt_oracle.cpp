@@ -227,6 +227,51 @@ void ktox(Orc* o, const C* in, double* out) {
 }
 
 // ---------------------------------------------------------------------------
+// shear.cu:467-527 ktox_shear, the lab-frame output transform:
+//   idft_shear_y (:481-513): for every stored kx row a direct one-sided sum over
+//     ky >= 0 with weight 1 for ky = 0 and 2 otherwise, at the sheared
+//     wavenumber ky + ky_shift[kx], in the reference's operation order;
+//   pad2d (:515-527): kx rows into nx rows, zero between nkxh and nkxh2;
+//   ktox_1d (fft.cu:208-250): C2C inverse along x per y, keep the real part.
+// ---------------------------------------------------------------------------
+void ktox_shear(Orc* o, const C* in, double* out) {
+    const int nx = o->nx, ny = o->ny, nkx = o->nkx, nky = o->nky;
+    std::vector<C> ctmp2((size_t)nkx * ny);
+    #pragma omp parallel for num_threads(orc_threads(o)) schedule(static)
+    for (int xid = 0; xid < nkx; ++xid) {
+        const double ky_shift = o->ky_shift[xid];
+        for (int yid = 0; yid < ny; ++yid) {
+            const double y_data = o->yy[yid];
+            const C* f = in + (size_t)xid * nky;
+            double ky = o->ky[0] + ky_shift;
+            C out_data;
+            out_data.x = f[0].x * cos(ky * y_data) - f[0].y * sin(ky * y_data);
+            out_data.y = f[0].y * cos(ky * y_data) + f[0].x * sin(ky * y_data);
+            for (int iky = 1; iky < nky; ++iky) {
+                ky = o->ky[iky] + ky_shift;
+                out_data.x += 2 * f[iky].x * cos(ky * y_data) - 2 * f[iky].y * sin(ky * y_data);
+                out_data.y += 2 * f[iky].y * cos(ky * y_data) + 2 * f[iky].x * sin(ky * y_data);
+            }
+            ctmp2[(size_t)xid * ny + yid] = out_data;
+        }
+    }
+    #pragma omp parallel num_threads(orc_threads(o))
+    {
+        std::vector<C> col(nx), tmp(nx);
+        #pragma omp for schedule(static)
+        for (int yid = 0; yid < ny; ++yid) {
+            for (int xid = 0; xid < nx; ++xid) {   // pad2d
+                if (xid < o->nkxh) col[xid] = ctmp2[(size_t)xid * ny + yid];
+                else if (xid > o->nkxh2) col[xid] = ctmp2[(size_t)(xid - o->nkxpad) * ny + yid];
+                else col[xid] = CZERO;
+            }
+            fft_exec(o->px, col.data(), +1, tmp.data());
+            for (int xid = 0; xid < nx; ++xid) out[(size_t)xid * ny + yid] = col[xid].x;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // fourier.cu:170-240 and shear.cu:326-374 -- pointwise spectral operators.
 // ---------------------------------------------------------------------------
 void negative(Orc* o, double* f) {                      // fourier.cu:170-176
@@ -711,6 +756,10 @@ void orc_xtok(void* h, const double* in, double* out) {
 }
 void orc_ktox(void* h, const double* in, double* out) {
     ktox(static_cast<Orc*>(h), reinterpret_cast<const C*>(in), out);
+}
+// ktox_shear (shear.cu:467-478) with the oracle's current ky_shift
+void orc_ktox_shear(void* h, const double* in, double* out) {
+    ktox_shear(static_cast<Orc*>(h), reinterpret_cast<const C*>(in), out);
 }
 void orc_shearing_field(void* h, const int* jmp, const double* in, double* out) {
     shearing_field(static_cast<Orc*>(h), jmp, reinterpret_cast<const C*>(in), reinterpret_cast<C*>(out));

@@ -204,6 +204,89 @@ k_aux_xinv(AuxArgs<T> a) {
     for (int m = 0; m < P; ++m) f[(long long)(tid + m * TT) * g.ny + y] = v[m].x;
 }
 
+// Lab-frame output transform ktox_shear (shear.cu:467-478), y part
+// (idft_shear_y, :481-513): for every stored kx row (both signs) the one-sided
+// ky sum with weights 1 (ky = 0) and 2 (ky > 0) at the sheared wavenumber
+// ky + ky_shift[kx].  Here it is an inverse FFT of length NY of the weighted
+// row (zero for ky >= nky) followed by the phase exp(i ky_shift[kx] yy[iy]),
+// yy[iy] = iy*dy, formed in FP64.  Output tiles [t][row][c] (y = 4t + c), row
+// stride nkx.
+template <typename T, int NY>
+__global__ void __launch_bounds__(kTile * TeamShape<NY>::TT)
+k_aux_yinv_lab(AuxArgs<T> a) {
+    using S = TeamShape<NY>;
+    constexpr int P = S::P, TT = S::TT, L = lds_len<NY>();
+    constexpr int LP = stage_stride<NY>();
+    __shared__ cplx<T> sm[kTile * L];
+    const Geo& g = a.g;
+    const int team = threadIdx.x / TT, tid = threadIdx.x - team * TT;
+    const int tile = blockIdx.x, row = tile * kTile + team;
+    const bool act = row < g.nkx;
+    const int nrow_tile = min(kTile, g.nkx - tile * kTile);
+    const cplx<T>* F = static_cast<const cplx<T>*>(a.in);
+    cplx<T>* lds = sm + team * L;
+    auto sync = [] { __syncthreads(); };
+    cplx<T> v[P];
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+        const int e = tid + m * TT;
+        cplx<T> z = czero<T>();
+        if (act && e < g.nky) {
+            z = view_load(F, row, e, g, a.jmp);
+            if (e > 0) z = {T(2) * z.x, T(2) * z.y};
+        }
+        v[m] = z;
+    }
+    TeamFFT<T, NY, P, S::RMAX, +1>::run(v, lds, a.tw, tid, sync);
+    const double sh = act ? a.s[row] : 0.0;
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+        double sn, cs;
+        sincos(__dmul_rn(sh, (double)(tid + m * TT) * g.dy), &sn, &cs);
+        v[m] = cmul(v[m], cplx<T>{(T)cs, (T)sn});
+    }
+#pragma unroll
+    for (int m = 0; m < P; ++m) sm[team * LP + tid + m * TT] = v[m];
+    __syncthreads();
+    cplx<T>* Gl = static_cast<cplx<T>*>(a.out);
+    for (int idx = threadIdx.x; idx < NY * 4; idx += blockDim.x) {
+        const int t = idx >> 4, w = idx & 15;
+        if ((w >> 2) < nrow_tile)
+            Gl[((long long)t * g.nkx + tile * kTile) * 4 + w] = sm[(w >> 2) * LP + 4 * t + (w & 3)];
+    }
+}
+
+// Lab-frame output transform, x part (pad2d, shear.cu:515-527, and ktox_1d,
+// fft.cu:208-250): per y column, the stored kx rows padded to NX, a complex
+// inverse FFT, and the real part -> f[ix*ny + iy].
+template <typename T, int NX>
+__global__ void __launch_bounds__(kTile * TeamShape<NX>::TT)
+k_aux_xinv_lab(AuxArgs<T> a) {
+    using S = TeamShape<NX>;
+    constexpr int P = S::P, TT = S::TT, L = lds_len<NX>();
+    __shared__ cplx<T> sm[kTile * L];
+    const Geo& g = a.g;
+    const int team = threadIdx.x / TT, tid = threadIdx.x - team * TT;
+    const int tile = blockIdx.x, y = tile * kTile + team, nk = g.nkx;
+    const cplx<T>* Gl = static_cast<const cplx<T>*>(a.in) + (long long)tile * nk * 4;
+    T* f = static_cast<T*>(a.out);
+    cplx<T>* lds = sm + team * L;
+    auto sync = [] { __syncthreads(); };
+    for (int idx = threadIdx.x; idx < nk * 4; idx += blockDim.x) sm[(idx & 3) * nk + (idx >> 2)] = Gl[idx];
+    __syncthreads();
+    cplx<T> v[P];
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+        const int e = tid + m * TT;
+        const int i = e < g.nkxh ? e : (e > g.nkxh2 ? e - g.nkxpad : -1);
+        v[m] = i >= 0 ? sm[team * nk + i] : czero<T>();
+    }
+    __syncthreads();
+    TeamFFT<T, NX, P, S::RMAX, +1>::run(v, lds, a.tw, tid, sync);
+#pragma unroll
+    for (int m = 0; m < P; ++m) f[(long long)(tid + m * TT) * g.ny + y] = v[m].x;
+}
+
 // out[t] = remap view of in (shear.cu:296-322); jmp null -> plain copy.
 template <typename T>
 __global__ void k_view_copy(Geo g, const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out,

@@ -20,6 +20,8 @@ struct Launch {
     static hipError_t yfwd(const AuxArgs<T>& a, hipStream_t s);
     static hipError_t yinv(const AuxArgs<T>& a, hipStream_t s);
     static hipError_t xinv(const AuxArgs<T>& a, hipStream_t s);
+    static hipError_t yinv_lab(const AuxArgs<T>& a, hipStream_t s);
+    static hipError_t xinv_lab(const AuxArgs<T>& a, hipStream_t s);
     static hipError_t view_copy(const Geo& g, const void* in, void* out, const int* jmp, hipStream_t s);
     static hipError_t phi_from_omega(const Geo& g, const void* om, void* phi, const double* kx,
                                      const double* ky, const double* sh, hipStream_t s);

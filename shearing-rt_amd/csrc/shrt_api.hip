@@ -254,6 +254,20 @@ hipError_t ktox_dev(shrt_ctx* c, const void* spec, void* phys, const double* s, 
     return c->prec == 4 ? ktox_dev<float>(c, spec, phys, s, jmp, mode) : ktox_dev<double>(c, spec, phys, s, jmp, mode);
 }
 
+// Lab-frame ktox_shear on device buffers (spec -> phys) with the shift `s` and
+// an optional remap view; the nkx x ny intermediate lives in G.
+template <typename T>
+hipError_t ktox_shear_dev(shrt_ctx* c, const void* spec, void* phys, const double* s, const int* jmp) {
+    AuxArgs<T> a = aux_args<T>(c, spec, c->G, s, jmp, false, 0);
+    hipError_t e = Launch<T>::yinv_lab(a, c->stream);
+    if (e != hipSuccess) return e;
+    AuxArgs<T> b = aux_args<T>(c, c->G, phys, s, nullptr, true, 0);
+    return Launch<T>::xinv_lab(b, c->stream);
+}
+hipError_t ktox_shear_dev(shrt_ctx* c, const void* spec, void* phys, const double* s, const int* jmp) {
+    return c->prec == 4 ? ktox_shear_dev<float>(c, spec, phys, s, jmp) : ktox_shear_dev<double>(c, spec, phys, s, jmp);
+}
+
 hipError_t zero_buffer(shrt_ctx* c, void* p, long long nelem) {
     return c->prec == 4 ? Launch<float>::fill_zero(p, nelem, c->stream)
                         : Launch<double>::fill_zero(p, nelem, c->stream);
@@ -782,7 +796,8 @@ shrt_status shrt_set_buffer(shrt_ctx* c, int ring, int j, const void* host) {
 
 shrt_status shrt_get_physical(shrt_ctx* c, int pf, int lab_frame, void* host) {
     if (!c || !host) return fail(SHRT_ERR_INVALID, "null argument");
-    if (lab_frame) return fail(SHRT_ERR_UNSUPPORTED, "lab-frame output (ktox_shear) is not implemented");
+    if (lab_frame && (pf == SHRT_P_VX || pf == SHRT_P_VY))
+        return fail(SHRT_ERR_INVALID, "velocities are output in the sheared frame only");
     StepCtl v;
     shrt_status st = read_ctl(c, &v);
     if (st != SHRT_OK) return st;
@@ -802,7 +817,9 @@ shrt_status shrt_get_physical(shrt_ctx* c, int pf, int lab_frame, void* host) {
     }
     const int* jmp = nullptr;
     if (mode == 0 && (st = view_for(c, v, ep, &jmp)) != SHRT_OK) return st;
-    HIP_TRY(ktox_dev(c, src, c->tmp_phys, s, jmp, mode));
+    // output_fields (file_access.cu:250-259): the lab frame when sigma != 0
+    if (lab_frame && c->ph.sigma != 0.0) HIP_TRY(ktox_shear_dev(c, src, c->tmp_phys, s_now, jmp));
+    else HIP_TRY(ktox_dev(c, src, c->tmp_phys, s, jmp, mode));
     HIP_TRY(hipMemcpyAsync(host, c->tmp_phys, phys_bytes(c), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return SHRT_OK;
@@ -821,6 +838,15 @@ shrt_status shrt_ktox(shrt_ctx* c, const void* spec_in, void* phys_out) {
     if (!c || !spec_in || !phys_out) return fail(SHRT_ERR_INVALID, "null argument");
     HIP_TRY(hipMemcpyAsync(c->tmp_spec, spec_in, spec_bytes(c), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(ktox_dev(c, c->tmp_spec, c->tmp_phys, c->s + (size_t)c->cur * c->g.nkx, nullptr, 0));
+    HIP_TRY(hipMemcpyAsync(phys_out, c->tmp_phys, phys_bytes(c), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return SHRT_OK;
+}
+
+shrt_status shrt_ktox_shear(shrt_ctx* c, const void* spec_in, void* phys_out) {
+    if (!c || !spec_in || !phys_out) return fail(SHRT_ERR_INVALID, "null argument");
+    HIP_TRY(hipMemcpyAsync(c->tmp_spec, spec_in, spec_bytes(c), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(ktox_shear_dev(c, c->tmp_spec, c->tmp_phys, c->s + (size_t)c->cur * c->g.nkx, nullptr));
     HIP_TRY(hipMemcpyAsync(phys_out, c->tmp_phys, phys_bytes(c), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return SHRT_OK;

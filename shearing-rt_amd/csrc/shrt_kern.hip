@@ -82,9 +82,12 @@ hipError_t Launch<R>::p2(const StepArgs<R>& a, int grid, hipStream_t s) {
 
 template <>
 hipError_t Launch<R>::p3(const StepArgs<R>& a, int grid, hipStream_t s) {
+    (void)grid;
     bool ok = dispatch(a.g.ny, [&](auto nc) {
         constexpr int N = decltype(nc)::value;
-        hipLaunchKernelGGL((k_p3<R, N>), dim3(grid), dim3(kTile * TeamShape<N>::TT), 0, s, a);
+        constexpr int KT = p3_teams<R, N>();
+        const int groups = (a.g.nkxh + KT - 1) / KT;
+        hipLaunchKernelGGL((k_p3<R, N>), dim3(2 * groups), dim3(KT * TeamShape<N>::TT), 0, s, a);
     });
     return ok ? hipGetLastError() : unsupported();
 }
@@ -121,6 +124,25 @@ hipError_t Launch<R>::xinv(const AuxArgs<R>& a, hipStream_t s) {
     bool ok = dispatch(a.g.nx, [&](auto nc) {
         constexpr int N = decltype(nc)::value;
         hipLaunchKernelGGL((k_aux_xinv<R, N>), dim3(a.g.ntile_y), dim3(kTile * TeamShape<N>::TT), 0, s, a);
+    });
+    return ok ? hipGetLastError() : unsupported();
+}
+
+template <>
+hipError_t Launch<R>::yinv_lab(const AuxArgs<R>& a, hipStream_t s) {
+    bool ok = dispatch(a.g.ny, [&](auto nc) {
+        constexpr int N = decltype(nc)::value;
+        hipLaunchKernelGGL((k_aux_yinv_lab<R, N>), dim3((a.g.nkx + kTile - 1) / kTile),
+                           dim3(kTile * TeamShape<N>::TT), 0, s, a);
+    });
+    return ok ? hipGetLastError() : unsupported();
+}
+
+template <>
+hipError_t Launch<R>::xinv_lab(const AuxArgs<R>& a, hipStream_t s) {
+    bool ok = dispatch(a.g.nx, [&](auto nc) {
+        constexpr int N = decltype(nc)::value;
+        hipLaunchKernelGGL((k_aux_xinv_lab<R, N>), dim3(a.g.ntile_y), dim3(kTile * TeamShape<N>::TT), 0, s, a);
     });
     return ok ? hipGetLastError() : unsupported();
 }

@@ -23,11 +23,11 @@ constexpr int kTile = 4;   // rows (kx) per P1/P3 workgroup == columns (y) per P
 // Waves per SIMD that the LDS of a kTile-team workgroup allows (160 KiB per
 // CU, four SIMDs), capped at 4 (128 VGPRs).  Used as the occupancy target
 // of the transform kernels so the compiler keeps their registers within it.
-template <typename T, int N, int TT = TeamShape<N>::TT>
+template <typename T, int N, int TT = TeamShape<N>::TT, int TEAMS = kTile>
 constexpr int waves_per_eu() {
-    constexpr long long lds = (long long)kTile * lds_len<N>() * (long long)sizeof(cplx<T>);
+    constexpr long long lds = (long long)TEAMS * lds_len<N>() * (long long)sizeof(cplx<T>);
     constexpr long long wgs = (160LL * 1024LL) / lds;
-    constexpr long long waves = wgs * ((kTile * TT + 63) / 64);
+    constexpr long long waves = wgs * ((TEAMS * TT + 63) / 64);
     constexpr long long w = waves / 4;
     return w < 1 ? 1 : (w > 4 ? 4 : (int)w);
 }
@@ -36,6 +36,8 @@ constexpr int waves_per_eu() {
 template <typename T, int N> constexpr int p1_points() { return TeamShape<N>::P; }
 template <typename T, int N> constexpr int p2_points() { return TeamShape<N>::P; }
 template <typename T, int N> constexpr int p3_points() { return TeamShape<N>::P; }
+// Row pairs (teams) per P3 workgroup: 1 or kTile.
+template <typename T, int N> constexpr int p3_teams() { return 1; }
 
 // Exchange synchronisation of a team: a wave-level fence for one-wave teams,
 // otherwise a workgroup barrier.

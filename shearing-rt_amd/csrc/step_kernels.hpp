@@ -506,28 +506,34 @@ __device__ void p3_bookkeeping(const StepArgs<T>& a, double dt, int h, int which
     }
 }
 
+// Workgroups of P3: (group of KT row pairs, field), KT = p3_teams.  One row
+// pair per workgroup (KT = 1) gives eight small workgroups per CU instead of
+// two large ones, so their load, transform and update phases overlap.
 template <typename T, int NY>
-__global__ void __launch_bounds__(kTile * TeamShape<NY>::TT, (waves_per_eu<T, NY>()))
+__global__ void __launch_bounds__((p3_teams<T, NY>() * TeamShape<NY>::TT),
+                                  (waves_per_eu<T, NY, TeamShape<NY>::TT, p3_teams<T, NY>()>()))
 k_p3(StepArgs<T> a) {
     using S = TeamShape<NY>;
+    constexpr int KT = p3_teams<T, NY>();
     constexpr int P = S::P, TT = S::TT, L = lds_len<NY>();
-    constexpr int NT = kTile * TT;
+    constexpr int NT = KT * TT;
     constexpr int LP = stage_stride<NY>();
     constexpr int J = (NY / 3 + 1 + TT - 1) / TT;   // row points per thread (nky <= NY/3 + 1)
     constexpr int JB = sizeof(T) == 4 ? 3 : 2;      // points whose loads are in flight together
-    __shared__ cplx<T> sm[kTile * L];
+    static_assert(KT == 1 || KT == kTile, "P3 workgroups hold one row pair or one H tile");
+    __shared__ cplx<T> sm[KT * L];
     __shared__ double red[32];
     const Geo& g = a.g;
     const Phys& ph = a.ph;
     const int team = threadIdx.x / TT, tid = threadIdx.x - team * TT;
-    // the omega and rho workgroups of a tile read each other's rows: keep them on one XCD
+    // the omega and rho workgroups of a row group read each other's rows: keep them on one XCD
     const int q = xcd_item(blockIdx.x, gridDim.x);
     const int tile = q >> 1;
     const int f = q & 1;   // 0: omega, 1: rho
-    const int row = tile * kTile + team;
+    const int row = tile * KT + team;
     const bool act = row < g.nkxh;
     const int rn = row == 0 ? 0 : g.nkx - row;
-    const int nrow_tile = min(kTile, g.nkxh - tile * kTile);
+    const int nrow_tile = min(KT, g.nkxh - tile * KT);
     const StepCtl* c = a.ctl_in;
     const long long n = c->istep;
     cplx<T>* lds = sm + team * L;
@@ -636,24 +642,33 @@ k_p3(StepArgs<T> a) {
             typename FFT::Twiddles ts;
             ts.load(a.twy, tid);
             const cplx<T>* Hf = a.H + (long long)f * g.I;
-            cplx<T> hv[P];
-#pragma unroll
-            for (int m = 0; m < P; ++m) {   // NY*4 tile elements over NT threads
-                const int idx = threadIdx.x + m * NT;
-                const int t = idx >> 4, w = idx & 15;
-                hv[m] = (w >> 2) < nrow_tile ? Hf[((long long)t * g.nkxh + tile * kTile) * 4 + w] : czero<T>();
-            }
-#pragma unroll
-            for (int m = 0; m < P; ++m) {
-                const int idx = threadIdx.x + m * NT;
-                const int t = idx >> 4, w = idx & 15;
-                sm[(w >> 2) * LP + 4 * t + (w & 3)] = hv[m];
-            }
-            __syncthreads();
             cplx<T> v[P];
+            if constexpr (KT == 1) {
+                // this row of the [t][kx][c] tiles: runs of 4 (32 bytes) per t
 #pragma unroll
-            for (int m = 0; m < P; ++m) v[m] = sm[team * LP + tid + m * TT];
-            __syncthreads();
+                for (int m = 0; m < P; ++m) {
+                    const int y = tid + m * TT;
+                    v[m] = Hf[((long long)(y >> 2) * g.nkxh + row) * 4 + (y & 3)];
+                }
+            } else {
+                cplx<T> hv[P];
+#pragma unroll
+                for (int m = 0; m < P; ++m) {   // NY*4 tile elements over NT threads
+                    const int idx = threadIdx.x + m * NT;
+                    const int t = idx >> 4, w = idx & 15;
+                    hv[m] = (w >> 2) < nrow_tile ? Hf[((long long)t * g.nkxh + tile * KT) * 4 + w] : czero<T>();
+                }
+#pragma unroll
+                for (int m = 0; m < P; ++m) {
+                    const int idx = threadIdx.x + m * NT;
+                    const int t = idx >> 4, w = idx & 15;
+                    sm[(w >> 2) * LP + 4 * t + (w & 3)] = hv[m];
+                }
+                __syncthreads();
+#pragma unroll
+                for (int m = 0; m < P; ++m) v[m] = sm[team * LP + tid + m * TT];
+                __syncthreads();
+            }
             // the first operands travel during the transform (loads complete in
             // order, so they are issued after the H tile has been consumed)
             load_batch(view, std::integral_constant<int, 0>{}, bt[0]);

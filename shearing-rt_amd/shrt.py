@@ -81,6 +81,7 @@ _SIGS = {
     "shrt_get_physical": ([_VP, ctypes.c_int, ctypes.c_int, _VP], ctypes.c_int),
     "shrt_xtok": ([_VP, _VP, _VP], ctypes.c_int),
     "shrt_ktox": ([_VP, _VP, _VP], ctypes.c_int),
+    "shrt_ktox_shear": ([_VP, _VP, _VP], ctypes.c_int),
     "shrt_shearing_field": ([_VP, _VP, _VP, _VP], ctypes.c_int),
     "shrt_get_time": ([_VP, ctypes.POINTER(Time)], ctypes.c_int),
     "shrt_set_time": ([_VP, ctypes.c_int64, ctypes.c_double, ctypes.c_double], ctypes.c_int),
@@ -223,6 +224,12 @@ class Shrt:
         a = np.ascontiguousarray(spec, dtype=self.cplx).reshape(self.shape_k)
         out = np.zeros((self.nx, self.ny), dtype=self.real)
         _check(lib().shrt_ktox(self.h, _ptr(a), _ptr(out)))
+        return out
+
+    def ktox_shear(self, spec):
+        a = np.ascontiguousarray(spec, dtype=self.cplx).reshape(self.shape_k)
+        out = np.zeros((self.nx, self.ny), dtype=self.real)
+        _check(lib().shrt_ktox_shear(self.h, _ptr(a), _ptr(out)))
         return out
 
     def shearing_field(self, jump, spec):

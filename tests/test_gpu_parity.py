@@ -261,3 +261,42 @@ def test_diagnostics_and_physical(gpu, prec):
         assert rel_l2(g.physical(pf), o.ktox(o.buffer(ring))) < TOLN[prec]
     assert rel_l2(g.physical(shrt.P_VX), o.physical("vx")) < TOLN[prec]
     assert rel_l2(g.physical(shrt.P_VY), o.physical("vy")) < TOLN[prec]
+
+
+@pytest.mark.parametrize("prec", [8, 4])
+@pytest.mark.parametrize("nx,ny", [(32, 32), (64, 64), (64, 128), (256, 256)])
+def test_ktox_shear(gpu, prec, nx, ny):
+    """Lab-frame output transform with a random shift against the oracle's
+    direct sheared sum."""
+    print(f"ktox_shear nx={nx} ny={ny} prec={prec}", flush=True)
+    o, g = pair(nx, prec, ny=ny, sigma=1.0)
+    nkx, nky = o.shape_k
+    kx, _ = o.k_tables()
+    rng = np.random.default_rng(nx * 3 + ny)
+    s = rng.uniform(-0.9, 0.9, nkx) * kx[1]
+    s[0] = 0.0
+    o.set_shift(s)
+    g.set_shift(s)
+    S = random_spectrum(rng, nkx, nky, decay=0.5, amp=1.0)
+    assert rel_l2(g.ktox_shear(S), o.ktox_shear(S)) < TOL1[prec] * 5
+
+
+@pytest.mark.parametrize("prec", [8, 4])
+def test_lab_frame_output(gpu, prec):
+    """What output_fields writes when sigma != 0: omega, phi, rho in the lab
+    frame after a run, including right after a remap (pending view)."""
+    n = 32
+    print(f"lab frame output prec={prec}", flush=True)
+    o, g = pair(n, prec, sigma=1.0, delt=0.01, cfl_num=5.0, nu=1e-2, kappa=1e-2)
+    ic = dict(seed=17, kmin=1.0, kmax=6.0, amp=1e-3, expo=-1.0)
+    o.set_spectral("rho", oracle.seeded_spectrum(n, n, 2 * np.pi, 2 * np.pi, **ic))
+    g.initialize_seeded(**ic)
+    for steps in (37, 63):   # the second run ends on the remap step (index 99)
+        o.step(steps)
+        g.time_advance(steps)
+        for pf, ring in ((shrt.P_OMEGA, "aomg"), (shrt.P_PHI, "aphi"), (shrt.P_RHO, "arho")):
+            err = rel_l2(g.physical(pf, lab_frame=1), o.ktox_shear(o.buffer(ring)))
+            assert err < TOLN[prec], (steps, pf, err)
+    assert g.state()["remap_count"] == 1
+    with pytest.raises(shrt.ShrtError):
+        g.physical(shrt.P_VX, lab_frame=1)
