@@ -13,6 +13,7 @@ template <typename T>
 struct Launch {
     static bool supported(int n);
     static int team_threads(int n);
+    static int p2_workgroups(int nx, int ny);   // the P2 grid (number of CFL partials)
     static hipError_t p1(const StepArgs<T>& a, int grid, hipStream_t s);
     static hipError_t p2(const StepArgs<T>& a, int grid, hipStream_t s);
     static hipError_t p3(const StepArgs<T>& a, int grid, hipStream_t s);

@@ -65,7 +65,7 @@ struct shrt_ctx {
     int* jump = nullptr;      // [2][nkx]
     int* jump_tmp = nullptr;  // [nkx]
     HalvingRec* hlog = nullptr;
-    double* cflp = nullptr;   // [ntile_y][2] CFL partial maxima of the x pass
+    double* cflp = nullptr;   // [P2 workgroups][2] CFL partial maxima of the x pass
     double* diag_partial = nullptr;
     double* diag_out = nullptr;
     int cur = 0;
@@ -133,6 +133,7 @@ StepArgs<T> step_args(shrt_ctx* c, int in_par, int store_phi) {
     a.hlog_cap = kHalvingCap;
     a.store_phi = store_phi;
     a.cflp = c->cflp;
+    a.ncflp = Launch<T>::p2_workgroups(c->g.nx, c->g.ny);
     return a;
 }
 
@@ -490,7 +491,7 @@ shrt_status shrt_create(const shrt_params* p, shrt_ctx** out) {
     if ((st = dalloc(c, (void**)&c->jump, 2 * sizeof(int) * g.nkx)) != SHRT_OK) return bail(st);
     if ((st = dalloc(c, (void**)&c->jump_tmp, sizeof(int) * g.nkx)) != SHRT_OK) return bail(st);
     if ((st = dalloc(c, (void**)&c->hlog, sizeof(HalvingRec) * kHalvingCap)) != SHRT_OK) return bail(st);
-    if ((st = dalloc(c, (void**)&c->cflp, sizeof(double) * 2 * g.ntile_y)) != SHRT_OK) return bail(st);
+    if ((st = dalloc(c, (void**)&c->cflp, sizeof(double) * 2 * g.ny)) != SHRT_OK) return bail(st);   // <= one per column
     if ((st = dalloc(c, (void**)&c->diag_partial, sizeof(double) * 2 * kDiagQ * kDiagBlocks)) != SHRT_OK)
         return bail(st);
     if ((st = dalloc(c, (void**)&c->diag_out, sizeof(double) * 4)) != SHRT_OK) return bail(st);

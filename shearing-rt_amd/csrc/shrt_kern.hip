@@ -54,6 +54,13 @@ bool Launch<R>::supported(int n) {
 }
 
 template <>
+int Launch<R>::p2_workgroups(int nx, int ny) {
+    int w = 0;
+    dispatch(nx, [&](auto nc) { w = ny / p2_teams<R, decltype(nc)::value>(); });
+    return w;
+}
+
+template <>
 int Launch<R>::team_threads(int n) {
     int tt = 0;
     dispatch(n, [&](auto nc) { tt = TeamShape<decltype(nc)::value>::TT; });
@@ -73,9 +80,11 @@ hipError_t Launch<R>::p1(const StepArgs<R>& a, int grid, hipStream_t s) {
 
 template <>
 hipError_t Launch<R>::p2(const StepArgs<R>& a, int grid, hipStream_t s) {
+    (void)grid;
     bool ok = dispatch(a.g.nx, [&](auto nc) {
         constexpr int N = decltype(nc)::value;
-        hipLaunchKernelGGL((k_p2<R, N>), dim3(grid), dim3(kTile * Team<N, p2_points<R, N>()>::TT), 0, s, a);
+        constexpr int KT = p2_teams<R, N>();
+        hipLaunchKernelGGL((k_p2<R, N>), dim3(a.g.ny / KT), dim3(KT * Team<N, p2_points<R, N>()>::TT), 0, s, a);
     });
     return ok ? hipGetLastError() : unsupported();
 }

@@ -35,6 +35,8 @@ constexpr int waves_per_eu() {
 // Points per thread of each step kernel.
 template <typename T, int N> constexpr int p1_points() { return TeamShape<N>::P; }
 template <typename T, int N> constexpr int p2_points() { return TeamShape<N>::P; }
+// Columns (teams) per P2 workgroup: 1, 2 or kTile.
+template <typename T, int N> constexpr int p2_teams() { return 1; }
 template <typename T, int N> constexpr int p3_points() { return TeamShape<N>::P; }
 // Row pairs (teams) per P3 workgroup: 1 or kTile.
 template <typename T, int N> constexpr int p3_teams() { return 1; }
@@ -118,7 +120,8 @@ struct StepArgs {
     HalvingRec* hlog;
     int hlog_cap;
     int store_phi;
-    double* cflp;   // [ntile_y][2]: per-workgroup max|vx + sigma*y|, max|vy| of P2
+    double* cflp;   // [ncflp][2]: per-workgroup max|vx + sigma*y|, max|vy| of P2
+    int ncflp;      // P2 workgroups
 };
 
 // Wavefront max of a double.
@@ -132,10 +135,21 @@ __device__ __forceinline__ double wave_max(double v) {
 // workgroup does this; `red` is workgroup-shared scratch of 2*16 doubles).
 __device__ __forceinline__ void cfl_maxima(const double* __restrict__ cflp, int n, double* red, double* cvx,
                                            double* cvy) {
+    // the pairs as 16-byte loads, eight in flight per thread
+    const double2* p = reinterpret_cast<const double2*>(cflp);
     double m0 = 0.0, m1 = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        m0 = fmax(m0, cflp[2 * i]);
-        m1 = fmax(m1, cflp[2 * i + 1]);
+    int i = threadIdx.x;
+    for (; i + 7 * (int)blockDim.x < n; i += 8 * blockDim.x) {
+        double2 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = p[i + u * blockDim.x];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) { m0 = fmax(m0, v[u].x); m1 = fmax(m1, v[u].y); }
+    }
+    for (; i < n; i += blockDim.x) {
+        const double2 v = p[i];
+        m0 = fmax(m0, v.x);
+        m1 = fmax(m1, v.y);
     }
     m0 = wave_max(m0);
     m1 = wave_max(m1);
@@ -192,7 +206,7 @@ template <int N> constexpr int stage_stride() { return N >= 256 ? N + 8 : lds_le
 template <int NX> constexpr int p2_stage_stride() {
     constexpr int nk = NX / 3 + 1;
     constexpr int padded = (nk + 31) / 32 * 32 + 8;
-    return 8 * padded <= kTile * lds_len<NX>() ? padded : nk;
+    return 2 * padded <= lds_len<NX>() ? padded : nk;   // two fields per column
 }
 
 // Remapped read of a spectral buffer (shear.cu:296-322).  `jmp` is null when the

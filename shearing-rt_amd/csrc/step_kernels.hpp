@@ -236,23 +236,31 @@ k_p1(StepArgs<T> a) {
 // ---------------------------------------------------------------------------
 // P2
 // ---------------------------------------------------------------------------
+// Workgroups of P2: KT = p2_teams columns each (a quarter, half or all of a
+// G tile of kTile columns); the workgroups of one tile run on one XCD so the
+// tile's lines are fetched once into its L2.
 template <typename T, int NX>
-__global__ void __launch_bounds__((kTile * Team<NX, p2_points<T, NX>()>::TT),
-                                  (waves_per_eu<T, NX, Team<NX, p2_points<T, NX>()>::TT>()))
+__global__ void __launch_bounds__((p2_teams<T, NX>() * Team<NX, p2_points<T, NX>()>::TT),
+                                  (waves_per_eu<T, NX, Team<NX, p2_points<T, NX>()>::TT, p2_teams<T, NX>()>()))
 k_p2(StepArgs<T> a) {
     using S = Team<NX, p2_points<T, NX>()>;
+    constexpr int KT = p2_teams<T, NX>();
+    constexpr int SUB = kTile / KT;                    // workgroups per G tile
     constexpr int P = S::P, TT = S::TT, L = lds_len<NX>();
-    constexpr int NT = kTile * TT;
+    constexpr int NT = KT * TT;
     constexpr int NKMAX = NX / 3 + 1;                  // >= nkxh
     constexpr int NQ = (NKMAX + TT - 1) / TT;          // kx per thread in the split
-    constexpr int NS = (4 * NKMAX + NT - 1) / NT;      // staged elements per thread per field
+    constexpr int NS = (KT * NKMAX + NT - 1) / NT;     // staged elements per thread per field
     constexpr int KP = p2_stage_stride<NX>();
-    __shared__ cplx<T> sm[kTile * L];
+    static_assert(kTile % KT == 0, "P2 workgroups split a G tile evenly");
+    __shared__ cplx<T> sm[KT * L];
     __shared__ double red[32];
     const Geo& g = a.g;
     const int team = threadIdx.x / TT, tid = threadIdx.x - team * TT;
-    const int tile = blockIdx.x;
-    const int y = tile * kTile + team;
+    const int q = xcd_item(blockIdx.x, gridDim.x);
+    const int tile = q / SUB;
+    const int cbase = (q - tile * SUB) * KT;           // first column of this workgroup in the tile
+    const int y = tile * kTile + cbase + team;
     const int nk = g.nkxh;
     cplx<T>* lds = sm + team * L;
     const TeamSync<S::TT> sync{};
@@ -284,26 +292,26 @@ k_p2(StepArgs<T> a) {
         // stage the two tiles as [field][c][kx], all loads first
         cplx<T> ra[NS], rb[NS];
 #pragma unroll
-        for (int m = 0; m < NS; ++m) {
+        for (int m = 0; m < NS; ++m) {   // element idx: position idx / KT, column cbase + idx % KT
             const int idx = lane + m * NT;
-            const int id = idx < nk * 4 ? idx : 0;
-            ra[m] = GA[id];
-            rb[m] = GB[id];
+            const int gi = idx < nk * KT ? (idx / KT) * kTile + cbase + idx % KT : 0;
+            ra[m] = GA[gi];
+            rb[m] = GB[gi];
         }
         teams_join<TT>();
 #pragma unroll
         for (int m = 0; m < NS; ++m) {
             const int idx = lane + m * NT;
-            if (idx < nk * 4) {
-                sm[(idx & 3) * KP + (idx >> 2)] = ra[m];
-                sm[(4 + (idx & 3)) * KP + (idx >> 2)] = rb[m];
+            if (idx < nk * KT) {
+                sm[(idx % KT) * KP + idx / KT] = ra[m];
+                sm[(KT + idx % KT) * KP + idx / KT] = rb[m];
             }
         }
         __syncthreads();
         // Z(kx) = A + iB for kx >= 0, conj(A) + i conj(B) at nx - kx,
         // branch-free; real parts only at kx = 0
         const cplx<T>* ca = sm + team * KP;
-        const cplx<T>* cb = sm + (4 + team) * KP;
+        const cplx<T>* cb = sm + (KT + team) * KP;
         cplx<T> v[P];
 #pragma unroll
         for (int m = 0; m < P; ++m) {
@@ -355,8 +363,7 @@ k_p2(StepArgs<T> a) {
         __syncthreads();
         if (threadIdx.x == 0) {
             for (int i = 1; i < nw; ++i) { m0 = fmax(m0, red[i]); m1 = fmax(m1, red[16 + i]); }
-            a.cflp[2 * blockIdx.x] = m0;
-            a.cflp[2 * blockIdx.x + 1] = m1;
+            reinterpret_cast<double2*>(a.cflp)[q] = double2{m0, m1};
         }
     }
     if (a.ph.linear) return;
@@ -388,7 +395,7 @@ k_p2(StepArgs<T> a) {
         const int kx = tid + i * TT;
         if (kx < nk) {
             sm[team * KP + kx] = hw[i];
-            sm[(4 + team) * KP + kx] = hr[i];
+            sm[(KT + team) * KP + kx] = hr[i];
         }
     }
     __syncthreads();
@@ -397,9 +404,10 @@ k_p2(StepArgs<T> a) {
 #pragma unroll
     for (int m = 0; m < NS; ++m) {
         const int idx = threadIdx.x + m * NT;
-        if (idx < nk * 4) {
-            Hw[idx] = sm[(idx & 3) * KP + (idx >> 2)];
-            Hr[idx] = sm[(4 + (idx & 3)) * KP + (idx >> 2)];
+        if (idx < nk * KT) {
+            const int gi = (idx / KT) * kTile + cbase + idx % KT;
+            Hw[gi] = sm[(idx % KT) * KP + idx / KT];
+            Hr[gi] = sm[(KT + idx % KT) * KP + idx / KT];
         }
     }
 }
@@ -632,7 +640,7 @@ k_p3(StepArgs<T> a) {
         Batch bt[2];
 
         // the step size after the CFL halvings, and (workgroup 0) the bookkeeping
-        cfl_maxima(a.cflp, g.ntile_y, red, &cvx, &cvy);
+        cfl_maxima(a.cflp, a.ncflp, red, &cvx, &cvy);
         const int h = cfl_delt(c, g, ph, cvx, cvy, &dt, &which);
         if (blockIdx.x == 0) p3_bookkeeping(a, dt, h, which, cvx, cvy);
 
