@@ -72,8 +72,13 @@ hipError_t Launch<R>::p1(const StepArgs<R>& a, int grid, hipStream_t s) {
     (void)grid;
     bool ok = dispatch(a.g.ny, [&](auto nc) {
         constexpr int N = decltype(nc)::value;
-        const P1Grid pg(a.g.nkxh, a.ph.linear);
-        hipLaunchKernelGGL((k_p1<R, N>), dim3(pg.grid), dim3(kTile * TeamShape<N>::TT), 0, s, a);
+        if constexpr (p1_teams<R, N>() == 1) {
+            const int items = (a.g.nkxh - 1) * (a.ph.linear ? 2 : 6) + (a.ph.linear ? 1 : 3);
+            hipLaunchKernelGGL((k_p1_row<R, N>), dim3(items), dim3(TeamShape<N>::TT), 0, s, a);
+        } else {
+            const P1Grid pg(a.g.nkxh, a.ph.linear);
+            hipLaunchKernelGGL((k_p1<R, N>), dim3(pg.grid), dim3(kTile * TeamShape<N>::TT), 0, s, a);
+        }
     });
     return ok ? hipGetLastError() : unsupported();
 }

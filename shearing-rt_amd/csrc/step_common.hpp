@@ -34,6 +34,8 @@ constexpr int waves_per_eu() {
 
 // Points per thread of each step kernel.
 template <typename T, int N> constexpr int p1_points() { return TeamShape<N>::P; }
+// Rows (teams) per P1 workgroup: 1 (one transform per workgroup) or kTile.
+template <typename T, int N> constexpr int p1_teams() { return 1; }
 template <typename T, int N> constexpr int p2_points() { return TeamShape<N>::P; }
 // Columns (teams) per P2 workgroup: 1, 2 or kTile.
 template <typename T, int N> constexpr int p2_teams() { return 1; }

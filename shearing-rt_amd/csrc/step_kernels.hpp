@@ -233,6 +233,52 @@ k_p1(StepArgs<T> a) {
     }
 }
 
+// P1 with one (row, output) transform per workgroup of one team: eight
+// workgroups per CU that start and finish independently.  Item q < (nkxh-1)*nout
+// is row q/nout + 1 (G position q/nout), output q % nout; the last items are
+// row 0's transformed outputs, the first of which also writes its vanishing
+// ones.  Each team writes its G position directly, in runs of 4 (32 bytes) per
+// t; the outputs of neighbouring rows are dispatched together to one XCD, whose
+// L2 merges the runs into whole lines.
+template <typename T, int NY>
+__global__ void __launch_bounds__(TeamShape<NY>::TT, (waves_per_eu<T, NY, TeamShape<NY>::TT, 1>()))
+k_p1_row(StepArgs<T> a) {
+    using S = TeamShape<NY>;
+    constexpr int P = S::P, TT = S::TT, L = lds_len<NY>();
+    __shared__ cplx<T> sm[L];
+    const Geo& g = a.g;
+    const int tid = threadIdx.x;
+    const int nout = a.ph.linear ? 2 : 6;
+    const int nreg = (g.nkxh - 1) * nout;
+    const int q = xcd_item(blockIdx.x, gridDim.x);
+    const int pos = q < nreg ? q / nout : g.nkxh - 1;
+    const int oi = q < nreg ? q - pos * nout : p1_row0_output(q - nreg);
+    const int row = p1_row_of(pos, g.nkxh);
+    const TeamSync<S::TT> sync{};
+    cplx<T> v[P];
+    static_for<0, 6>([&](auto oc) {
+        constexpr int OI = decltype(oc)::value;
+        if (OI == oi) p1_spectrum<OI, T, NY>(a, row, true, tid, v);
+    });
+    TeamFFT<T, NY, P, S::RMAX, +1>::run(v, sm, a.twy, tid, sync);
+    cplx<T>* Gf = a.G + (long long)oi * g.I;
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+        const int y = tid + m * TT;
+        Gf[((long long)(y >> 2) * g.nkxh + pos) * 4 + (y & 3)] = v[m];
+    }
+    if (oi == 0 && a.store_phi) p1_store_phi<T, NY>(a, row, tid);
+    if (q == nreg) {
+        // outputs of row 0 that carry a factor kx = 0
+        const int nz = a.ph.linear ? 1 : 3;
+        for (int idx = tid; idx < nz * NY; idx += TT) {
+            const int z = idx / NY, y = idx - z * NY;
+            const int o = z == 0 ? 1 : (z == 1 ? 2 : 4);
+            a.G[(long long)o * g.I + ((long long)(y >> 2) * g.nkxh + pos) * 4 + (y & 3)] = czero<T>();
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // P2
 // ---------------------------------------------------------------------------
