@@ -92,8 +92,12 @@ typedef struct {
 typedef struct {
     int64_t istep;
     double time, value, delt;
-    int which;              /* 0 cfl_vx, 1 cfl_vy, 2 g/rho0, 3 rho0_prime */
+    int which;              /* 0 cfl_vx, 1 cfl_vy, 2 g/rho0, 3 rho0_prime (the last that halved) */
     int count;              /* halvings taken in that step                */
+    int counts[4];          /* halvings per condition, in check_cfl's order   */
+    double values[4];       /* cfl_vx, cfl_vy, g/rho0, rho0_prime of the step */
+    double delt_before;     /* delt entering the step; the printed lines of
+                               time_integral.cu:361-383 follow by halving it  */
 } shrt_halving;
 
 typedef struct shrt_ctx shrt_ctx;

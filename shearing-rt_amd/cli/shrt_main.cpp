@@ -1,0 +1,306 @@
+// shrt_main.cpp -- the shrt driver: the reference's main.cu loop (main.cu:13-76)
+// on the MI355X library, reading ./config.ini like input_data
+// (file_access.cu:123-165) and writing the reference's output files.
+//
+//   shrt [--config PATH] [--dble] [--graph-steps N] [--print-config]
+//
+// --dble selects FP64 (the reference's `make dble=on`); the default is FP32 as
+// in the reference's default build.  Between output times the driver runs
+// batches of steps on the device (one captured HIP graph per graph-steps
+// steps): delt only ever halves, so a batch of k steps with
+// time + (k+1)*delt below the next event cannot skip an event of the
+// reference loop.  The CFL lines the reference prints inside check_cfl
+// (time_integral.cu:361-383) are printed after each batch from the device's
+// halving log, in order.  An energy/enstrophy series is appended to energy.dat
+// at every field output (new; additive).
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/shrt.h"
+#include "ini.hpp"
+#include "writers.hpp"
+
+using namespace shrt_cli;
+
+namespace {
+
+struct Config {
+    int nx, ny, nthread;
+    double Lx, Ly, delt, tmax, cfl_num;
+    double output_time;
+    int nwrite;
+    bool write_fields;
+    bool noise_flag, linear_flag;
+    double nu, kappa, sigma, g, rho0_prime, rho0, rho_eps1, rho_eps2;
+};
+
+// input_data (file_access.cu:123-165): the same keys, defaults and order of
+// warnings.  With FP32 the floating values are read as float, as cureal is.
+Config read_config(const Ini& pt, bool dble) {
+    auto real = [&](const char* sec, const char* key, double def) -> double {
+        return dble ? pt.entry_double(sec, key, def, nullptr) : (double)pt.entry_float(sec, key, (float)def, nullptr);
+    };
+    Config c;
+    c.nx = pt.entry_int("simulation", "nx", 512, nullptr);
+    c.ny = pt.entry_int("simulation", "ny", 1024, nullptr);
+    c.nthread = pt.entry_int("simulation", "cuda thread num", 1024, nullptr);
+    c.Lx = real("simulation", "Lx", M_PI);
+    c.Ly = real("simulation", "Ly", M_PI);
+    c.delt = real("simulation", "time step", 1e-3);
+    c.tmax = real("simulation", "time max", 30);
+    c.cfl_num = real("simulation", "cfl number", 1e-1);
+    c.output_time = real("output", "output time step", 1.0);
+    c.nwrite = pt.entry_int("output", "output loop count", 100, nullptr);
+    c.write_fields = pt.entry_bool("output", "write output", true, nullptr);
+    c.noise_flag = pt.entry_bool("problem", "initial noise", true, nullptr);
+    c.linear_flag = pt.entry_bool("problem", "linear eq", false, nullptr);
+    c.nu = real("problem", "nu", 1e-3);
+    c.kappa = real("problem", "kappa", 1e-5);
+    c.sigma = real("problem", "sigma", 1.0);
+    c.g = real("problem", "g", 1.0);
+    c.rho0_prime = real("problem", "rho0_prime", 1.0);
+    c.rho0 = real("problem", "rho0", 1.0);
+    c.rho_eps1 = real("problem", "rho_eps1", 1e-2);
+    c.rho_eps2 = 0.1 * c.rho_eps1;
+    return c;
+}
+
+void print_config(const Config& c, bool dble) {
+    std::printf("nx = %d\nny = %d\ncuda thread num = %d\nLx = %.17g\nLy = %.17g\ntime step = %.17g\n"
+                "time max = %.17g\ncfl number = %.17g\noutput time step = %.17g\noutput loop count = %d\n"
+                "write output = %d\ninitial noise = %d\nlinear eq = %d\nnu = %.17g\nkappa = %.17g\n"
+                "sigma = %.17g\ng = %.17g\nrho0_prime = %.17g\nrho0 = %.17g\nrho_eps1 = %.17g\n"
+                "rho_eps2 = %.17g\nprecision = %s\n",
+                c.nx, c.ny, c.nthread, c.Lx, c.Ly, c.delt, c.tmax, c.cfl_num, c.output_time, c.nwrite,
+                (int)c.write_fields, (int)c.noise_flag, (int)c.linear_flag, c.nu, c.kappa, c.sigma, c.g,
+                c.rho0_prime, c.rho0, c.rho_eps1, c.rho_eps2, dble ? "fp64" : "fp32");
+}
+
+void check(shrt_status s, const char* what) {
+    if (s != SHRT_OK) {
+        std::fprintf(stderr, "shrt: %s failed: %s\n", what, shrt_last_error());
+        std::exit(1);
+    }
+}
+
+Grid make_grid(const Config& c) {
+    int sz[8];
+    check(shrt_sizes(c.nx, c.ny, sz), "shrt_sizes");
+    Grid g;
+    g.nx = c.nx;
+    g.ny = c.ny;
+    g.nkx = sz[0];
+    g.nky = sz[1];
+    g.nkxh = sz[2];
+    const double dx = c.Lx / c.nx, dy = c.Ly / c.ny;
+    g.xx.resize(c.nx + 1);
+    g.yy.resize(c.ny + 1);
+    for (int ix = 0; ix <= c.nx; ++ix) g.xx[ix] = ix * dx;   // time_integral.cu:236-237
+    for (int iy = 0; iy <= c.ny; ++iy) g.yy[iy] = iy * dy;
+    g.kx.assign(g.nkx, 0.0);
+    g.ky.assign(g.nky, 0.0);
+    for (int ikx = 0; ikx < g.nkxh; ++ikx) g.kx[ikx] = ikx * 2 * M_PI / c.Lx;   // fourier.cu:123-128
+    for (int ikx = 0; ikx < g.nkxh - 1; ++ikx) g.kx[g.nkx - ikx - 1] = -g.kx[ikx + 1];
+    for (int iky = 0; iky < g.nky; ++iky) g.ky[iky] = iky * 2 * M_PI / c.Ly;
+    g.kx_index.resize(g.nkx);   // file_access.cu:210-212
+    for (int ikx = g.nkxh; ikx < g.nkx; ++ikx) g.kx_index[ikx - g.nkxh] = ikx;
+    for (int ikx = 0; ikx < g.nkxh; ++ikx) g.kx_index[ikx + (g.nkx - g.nkxh)] = ikx;
+    return g;
+}
+
+template <typename R>
+struct Driver {
+    shrt_ctx* ctx;
+    const Config& c;
+    const Grid& g;
+    std::vector<R> ao, ap, ar;             // spectral, interleaved complex
+    std::vector<R> omg, phi, rho, vx, vy;  // physical
+    int printed_halvings = 0;
+    bool energy_started = false;
+
+    Driver(shrt_ctx* x, const Config& cc, const Grid& gg) : ctx(x), c(cc), g(gg) {
+        const size_t M = (size_t)g.nkx * g.nky, N = (size_t)g.nx * g.ny;
+        ao.resize(2 * M); ap.resize(2 * M); ar.resize(2 * M);
+        omg.resize(N); phi.resize(N); rho.resize(N); vx.resize(N); vy.resize(N);
+    }
+
+    shrt_time now() {
+        shrt_time t;
+        check(shrt_get_time(ctx, &t), "shrt_get_time");
+        return t;
+    }
+
+    void spectra() {
+        check(shrt_get_spectral(ctx, SHRT_OMEGA, ao.data()), "shrt_get_spectral");
+        check(shrt_get_spectral(ctx, SHRT_PHI, ap.data()), "shrt_get_spectral");
+        check(shrt_get_spectral(ctx, SHRT_RHO, ar.data()), "shrt_get_spectral");
+    }
+
+    // ktox or ktox_shear of omega, phi, rho (output_fields, output_maxamp)
+    void physical() {
+        check(shrt_get_physical(ctx, SHRT_P_OMEGA, 1, omg.data()), "shrt_get_physical");
+        check(shrt_get_physical(ctx, SHRT_P_PHI, 1, phi.data()), "shrt_get_physical");
+        check(shrt_get_physical(ctx, SHRT_P_RHO, 1, rho.data()), "shrt_get_physical");
+    }
+
+    void output_fields_now(int istep, double time) {
+        physical();
+        check(shrt_get_physical(ctx, SHRT_P_VX, 0, vx.data()), "shrt_get_physical");
+        check(shrt_get_physical(ctx, SHRT_P_VY, 0, vy.data()), "shrt_get_physical");
+        output_fields<R>(g, istep, c.nwrite, time, omg.data(), phi.data(), rho.data(), vx.data(), vy.data());
+        shrt_diag d;
+        check(shrt_diagnostics(ctx, &d), "shrt_diagnostics");
+        energy_series(!energy_started, time, d.energy, d.enstrophy, d.rho_var);
+        energy_started = true;
+    }
+
+    void output_maxamp_now(double time) {
+        physical();
+        const size_t N = omg.size();
+        output_maxamp<R>(time, max_abs(omg.data(), N), max_abs(phi.data(), N), max_abs(rho.data(), N));
+    }
+
+    void k_data_bef_now(double time, int istep) {
+        spectra();
+        k_data_bef<R>(g, time, istep, c.nwrite, ao.data(), ap.data(), ar.data());
+    }
+
+    // the CFL lines of check_cfl for halvings logged since the last call
+    void print_halvings() {
+        std::vector<shrt_halving> h(4096);
+        int n = 0;
+        check(shrt_get_halvings(ctx, h.data(), (int)h.size(), &n), "shrt_get_halvings");
+        static const char* names[4] = {"cfl_vx", "cfl_vy", "cfl_(g/rho0)", "cfl_rho0_prime"};
+        for (int i = printed_halvings; i < n; ++i) {
+            double d = h[i].delt_before;
+            for (int k = 0; k < 4; ++k)
+                for (int j = 0; j < h[i].counts[k]; ++j) {
+                    d /= 2.0;
+                    std::printf("istep = %d, time = %g, %s = %g, delt = %g\n", (int)h[i].istep, h[i].time, names[k],
+                                h[i].values[k], d);
+                }
+        }
+        printed_halvings = n;
+    }
+
+    int run() {
+        shrt_time t = now();
+        int istep = (int)t.istep;
+        double time = t.time;
+        double next_output_time = c.output_time;
+        const double output_maxamp_time = 0.1;
+        double next_output_maxamp_time = output_maxamp_time;
+
+        std::printf("yy[ny] = %lf\n", g.yy[g.ny]);
+        if (c.write_fields) {
+            output_fields_now(istep, time);
+            output_maxamp_now(time);
+            spectra();
+            en_spectral<R>(g, istep, time, ao.data(), ap.data(), ar.data());
+            ks_reim<R>(g, time, ao.data(), ap.data(), ar.data());
+            k_data_bef<R>(g, time, istep, c.nwrite, ao.data(), ap.data(), ar.data());
+            k_data_aft<R>(g, time, istep, c.nwrite, ao.data(), ap.data(), ar.data());
+        }
+
+        while (time <= c.tmax) {
+            // steps that cannot reach any event of the loop body
+            double horizon = c.tmax;
+            if (c.write_fields) {
+                horizon = std::fmin(horizon, next_output_time);
+                horizon = std::fmin(horizon, next_output_maxamp_time);
+            }
+            long long k = 1;
+            if (istep >= 1 && t.delt > 0) {
+                const double room = (horizon - time) / t.delt - 2.0;
+                if (room > 1.0) k = (long long)std::fmin(room, 1e6);
+            }
+            check(shrt_step(ctx, (int)k), "shrt_step");
+            t = now();
+            istep = (int)t.istep;
+            time = t.time;
+            print_halvings();
+
+            if (istep == 1) k_data_bef_now(time, istep);
+            if (c.write_fields && time > next_output_time) {
+                next_output_time += c.output_time;
+                output_fields_now(istep, time);
+                spectra();
+                en_spectral<R>(g, istep, time, ao.data(), ap.data(), ar.data());
+                ks_reim<R>(g, time, ao.data(), ap.data(), ar.data());
+                if (c.linear_flag || c.sigma == 0) k_data_bef<R>(g, time, istep, c.nwrite, ao.data(), ap.data(), ar.data());
+            }
+            if (c.write_fields && time > next_output_maxamp_time) {
+                next_output_maxamp_time += output_maxamp_time;
+                output_maxamp_now(time);
+            }
+        }
+        return 0;
+    }
+};
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    std::string path = "config.ini";
+    bool dble = false, print_only = false;
+    int graph_steps = 20;
+    for (int i = 1; i < argc; ++i) {
+        if (!std::strcmp(argv[i], "--config") && i + 1 < argc) path = argv[++i];
+        else if (!std::strcmp(argv[i], "--dble")) dble = true;
+        else if (!std::strcmp(argv[i], "--print-config")) print_only = true;
+        else if (!std::strcmp(argv[i], "--graph-steps") && i + 1 < argc) graph_steps = std::atoi(argv[++i]);
+        else {
+            std::fprintf(stderr, "usage: shrt [--config PATH] [--dble] [--graph-steps N] [--print-config]\n");
+            return 2;
+        }
+    }
+    Ini pt;
+    try {
+        pt = Ini::read(path);
+    } catch (const std::exception& e) {
+        std::printf("ERROR: unable to read config file: %s", e.what());
+        return 1;
+    }
+    const Config c = read_config(pt, dble);
+    if (print_only) {
+        print_config(c, dble);
+        return 0;
+    }
+
+    shrt_params p{};
+    p.nx = c.nx;
+    p.ny = c.ny;
+    p.Lx = c.Lx;
+    p.Ly = c.Ly;
+    p.delt = c.delt;
+    p.cfl_num = c.cfl_num;
+    p.nu = c.nu;
+    p.kappa = c.kappa;
+    p.sigma = c.sigma;
+    p.g = c.g;
+    p.rho0 = c.rho0;
+    p.rho0_prime = c.rho0_prime;
+    p.rho_eps1 = c.rho_eps1;
+    p.noise = c.noise_flag ? 1 : 0;
+    p.linear = c.linear_flag ? 1 : 0;
+    p.precision = dble ? SHRT_FP64 : SHRT_FP32;
+    p.graph_steps = graph_steps;
+    shrt_ctx* ctx = nullptr;
+    check(shrt_create(&p, &ctx), "shrt_create");
+    check(shrt_initialize(ctx), "shrt_initialize");
+    const Grid g = make_grid(c);
+    int rc;
+    if (dble) {
+        Driver<double> d(ctx, c, g);
+        rc = d.run();
+    } else {
+        Driver<float> d(ctx, c, g);
+        rc = d.run();
+    }
+    check(shrt_destroy(ctx), "shrt_destroy");
+    return rc;
+}

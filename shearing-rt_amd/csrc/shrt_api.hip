@@ -910,6 +910,11 @@ shrt_status shrt_get_halvings(shrt_ctx* c, shrt_halving* out, int cap, int* coun
         out[i].delt = h[i].delt;
         out[i].which = h[i].which;
         out[i].count = h[i].pad_;
+        for (int k = 0; k < 4; ++k) {
+            out[i].counts[k] = h[i].counts[k];
+            out[i].values[k] = h[i].values[k];
+        }
+        out[i].delt_before = h[i].delt_before;
     }
     *count = n;
     return SHRT_OK;

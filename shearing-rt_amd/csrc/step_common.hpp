@@ -97,6 +97,9 @@ struct HalvingRec {
     long long istep;
     double time, value, delt;
     int which, pad_;
+    int counts[4];      // halvings per condition, in check_cfl's order
+    double values[4];   // cfl_vx, cfl_vy, g/rho0, rho0_prime
+    double delt_before;
 };
 
 template <typename T>
@@ -240,17 +243,27 @@ __device__ __forceinline__ double k2_of(double kx, double kys) {
 
 // The step size after the CFL halvings of check_cfl (time_integral.cu:357-383).
 // Returns the halving count; `which` gets the last condition that halved.
+// `cnt` (may be null) receives the halvings per condition.
 __device__ __forceinline__ int cfl_delt(const StepCtl* c, const Geo& g, const Phys& ph, double cvx, double cvy,
-                                        double* dt_out, int* which) {
+                                        double* dt_out, int* which, int* cnt = nullptr) {
     double dt = c->delt;
     const double cg = ph.g / ph.rho0;
     const double cr = ph.rho0_prime;
-    int h = 0;
+    int h = 0, h0 = 0, h1 = 0, h2 = 0;
     *which = -1;
     while (__ddiv_rn(__dmul_rn(cvx, dt), g.dx) > ph.cfl_num && h < 2200) { dt /= 2.0; ++h; *which = 0; }
+    h0 = h;
     while (__ddiv_rn(__dmul_rn(cvy, dt), g.dy) > ph.cfl_num && h < 2200) { dt /= 2.0; ++h; *which = 1; }
+    h1 = h;
     while (__ddiv_rn(__dmul_rn(cg, dt), g.dx) > ph.cfl_num && h < 2200) { dt /= 2.0; ++h; *which = 2; }
+    h2 = h;
     while (__ddiv_rn(__dmul_rn(cr, dt), g.dx) > ph.cfl_num && h < 2200) { dt /= 2.0; ++h; *which = 3; }
+    if (cnt) {
+        cnt[0] = h0;
+        cnt[1] = h1 - h0;
+        cnt[2] = h2 - h1;
+        cnt[3] = h - h2;
+    }
     *dt_out = dt;
     return h;
 }

@@ -110,11 +110,12 @@ __device__ __forceinline__ void p1_spectrum(const StepArgs<T>& a, int row, bool 
     const int* jmp = (c->slot_epoch_a[s1] != c->epoch) ? a.jump + (c->epoch & 1) * g.nkx : nullptr;
     const double kxp = a.kx[rowc], kxn = a.kx[rn];
     const double sp = a.s_in[rowc], sn = a.s_in[rn];
+    constexpr int NKY = NY / 3 + 1;   // == nky for the supported (power-of-two) lengths
 #pragma unroll
     for (int m = 0; m < P; ++m) {
         const int e = tid + m * TT;
-        const bool pos = e < g.nky;
-        const bool keep = act && (pos || e > NY - g.nky);
+        const bool pos = e < NKY;
+        const bool keep = act && (pos || e > NY - NKY);
         const int iky = keep ? (pos ? e : NY - e) : 0;
         const int r = pos ? rowc : rn;
         const cplx<T> F = view_load(src, r, iky, g, jmp);
@@ -307,7 +308,7 @@ k_p2(StepArgs<T> a) {
     const int tile = q / SUB;
     const int cbase = (q - tile * SUB) * KT;           // first column of this workgroup in the tile
     const int y = tile * kTile + cbase + team;
-    const int nk = g.nkxh;
+    constexpr int nk = NKMAX;   // == nkxh for the supported (power-of-two) lengths
     cplx<T>* lds = sm + team * L;
     const TeamSync<S::TT> sync{};
 
@@ -330,7 +331,7 @@ k_p2(StepArgs<T> a) {
     for (int pass = 0; pass < npass; ++pass) {
         int lane = threadIdx.x;
         asm volatile("" : "+v"(lane));
-        const int tl = lane - team * TT;
+        const int tl = (lane - team * TT) & (TT - 1);
         const int fa = a.ph.linear ? 0 : (pass & 1);
         const int fb = a.ph.linear ? 1 : 2 + pass;
         const cplx<T>* GA = a.G + (long long)fa * g.I + (long long)tile * nk * 4;
@@ -554,6 +555,14 @@ __device__ void p3_bookkeeping(const StepArgs<T>& a, double dt, int h, int which
             rec.delt = dt;
             rec.which = which;
             rec.pad_ = h;
+            double dtc;
+            int wc;
+            cfl_delt(c, g, a.ph, cvx, cvy, &dtc, &wc, rec.counts);
+            rec.values[0] = cvx;
+            rec.values[1] = cvy;
+            rec.values[2] = a.ph.g / a.ph.rho0;
+            rec.values[3] = a.ph.rho0_prime;
+            rec.delt_before = c->delt;
             a.hlog[c->log_count] = rec;
             o->log_count = c->log_count + 1;
         }
@@ -572,7 +581,8 @@ k_p3(StepArgs<T> a) {
     constexpr int P = S::P, TT = S::TT, L = lds_len<NY>();
     constexpr int NT = KT * TT;
     constexpr int LP = stage_stride<NY>();
-    constexpr int J = (NY / 3 + 1 + TT - 1) / TT;   // row points per thread (nky <= NY/3 + 1)
+    constexpr int NKY = NY / 3 + 1;                 // == nky for the supported (power-of-two) lengths
+    constexpr int J = (NKY + TT - 1) / TT;          // row points per thread
     constexpr int JB = sizeof(T) == 4 ? 3 : 2;      // points whose loads are in flight together
     static_assert(KT == 1 || KT == kTile, "P3 workgroups hold one row pair or one H tile");
     __shared__ cplx<T> sm[KT * L];
@@ -641,7 +651,7 @@ k_p3(StepArgs<T> a) {
 #pragma unroll
         for (int u = 0; u < JB; ++u) {
             const int iky = tid + (j0 + u) * TT;
-            const int ik = (j0 + u < J && iky < g.nky) ? iky : 0;   // in bounds for idle lanes
+            const int ik = (j0 + u < J && iky < NKY) ? iky : 0;   // in bounds for idle lanes
             // unconditional: the first steps load history slots they then ignore
             B.x1[u] = ld(view, X1, r, ik, ja1);
             B.a1[u] = ld(view, A1, r, ik, ja1);
@@ -662,7 +672,7 @@ k_p3(StepArgs<T> a) {
 #pragma unroll
         for (int u = 0; u < JB; ++u) {
             const int iky = tid + (j0 + u) * TT;
-            if (j0 + u >= J || iky >= g.nky) continue;
+            if (j0 + u >= J || iky >= NKY) continue;
             cplx<T> nh = czero<T>();
             if (!ph.linear) {
                 const cplx<T> F = side ? cconj(lds[(NY - iky) & (NY - 1)]) : lds[iky];

@@ -61,7 +61,8 @@ class Diag(ctypes.Structure):
 
 class Halving(ctypes.Structure):
     _fields_ = [("istep", ctypes.c_int64), ("time", ctypes.c_double), ("value", ctypes.c_double),
-                ("delt", ctypes.c_double), ("which", ctypes.c_int), ("count", ctypes.c_int)]
+                ("delt", ctypes.c_double), ("which", ctypes.c_int), ("count", ctypes.c_int),
+                ("counts", ctypes.c_int * 4), ("values", ctypes.c_double * 4), ("delt_before", ctypes.c_double)]
 
 
 # Every entry point of include/shrt.h with its ctypes signature.
@@ -280,7 +281,8 @@ class Shrt:
         n = ctypes.c_int()
         _check(lib().shrt_get_halvings(self.h, arr, cap, ctypes.byref(n)))
         return [dict(istep=arr[i].istep, time=arr[i].time, value=arr[i].value, delt=arr[i].delt,
-                     which=arr[i].which, count=arr[i].count) for i in range(n.value)]
+                     which=arr[i].which, count=arr[i].count, counts=list(arr[i].counts),
+                     values=list(arr[i].values), delt_before=arr[i].delt_before) for i in range(n.value)]
 
 
 def header_symbols():

@@ -1,0 +1,136 @@
+"""End-to-end: the shrt driver (main.cu's loop on the MI355X library) on a small
+config, against the reference's output schedule and file formats and against
+the oracle's fields at the output steps."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import oracle, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(REPO, "shearing-rt_amd", "shrt")
+
+CONFIG = """[simulation]
+nx = 32
+ny = 32
+cuda thread num = 1024
+time step = 1e-3
+time max = 0.0185
+cfl number = 0.5
+Lx = 3.1415926535897932384626433832795
+Ly = 3.1415926535897932384626433832795
+
+[problem]
+initial noise = true
+linear eq = false
+nu = 1e-3
+kappa = 8e-5
+sigma = 1.0
+g = 1.0
+rho0_prime = 1.0
+rho0 = 1.0
+rho_eps1 = 1e-2
+
+[output]
+write output = true
+output time step = 0.0075
+output loop count = 1
+"""
+
+
+def schedule():
+    """The output steps of main.cu:46-66 for this config, in FP64 as the run
+    computes them (time += delt, next_output_time += output time step)."""
+    t, nxt, steps = 0.0, 0.0075, []
+    while t <= 0.0185:
+        t += 1e-3
+        if t > nxt:
+            nxt += 0.0075
+            steps.append((round(t / 1e-3), "%09.6f" % t))
+    return steps
+
+
+OUT = schedule()   # [(8, '00.008000'), (15, '00.015000')]
+
+
+def read_table(path, ncol):
+    rows = [l.split() for l in open(path) if l.strip()]
+    a = np.array(rows, dtype=np.float64)
+    assert a.shape[1] == ncol
+    return a
+
+
+@pytest.fixture(scope="module")
+def run_dir(gpu, tmp_path_factory):
+    d = tmp_path_factory.mktemp("shrt_run")
+    with open(d / "config.ini", "w") as f:
+        f.write(CONFIG)
+    print("driver run in", d, flush=True)
+    p = subprocess.run([BIN, "--dble"], cwd=d, capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stdout + p.stderr
+    return d, p.stdout
+
+
+def test_output_schedule(run_dir):
+    d, out = run_dir
+    names = set(os.listdir(d))
+    assert OUT == [(8, "00.008000"), (15, "00.015000")]
+    for _, t in [(0, "00.000000")] + OUT:
+        assert f"vel_t{t}.dat" in names and f"ksre_t{t}.dat" in names and f"ksim_t{t}.dat" in names
+    assert {"n00000_t00.000000.dat", "n00008_t00.008000.dat", "n00015_t00.015000.dat"} <= names
+    assert len([n for n in names if n.startswith("n000")]) == 3
+    assert {"kx_ensp.txt", "ky_ensp.txt", "after_kdata.txt", "max_amp.dat", "energy.dat"} <= names
+    assert {"kx_ensp_t00.008000.dat", "ky_ensp_t00.015000.dat"} <= names
+    # k_data_bef at istep 0 and istep 1 only (sigma != 0, nonlinear)
+    assert sorted(n for n in names if n.startswith("befk")) == ["befk_n00000_t00.000000.dat",
+                                                                 "befk_n00001_t00.001000.dat"]
+    assert not any(n.startswith("aftk") for n in names)
+    assert out.startswith("yy[ny] = 3.141593\n")
+    assert out.count("time = ") == 3
+    assert len(open(d / "max_amp.dat").read().splitlines()) == 1
+    assert len(open(d / "energy.dat").read().splitlines()) == 3
+
+
+def test_field_files_match_oracle(run_dir):
+    d, _ = run_dir
+    o = oracle.Oracle(nx=32, ny=32, Lx=np.pi, Ly=np.pi, delt=1e-3, cfl_num=0.5, nu=1e-3, kappa=8e-5,
+                      sigma=1.0, g=1.0, rho0=1.0, rho0_prime=1.0, rho_eps1=1e-2, noise=1, linear=0)
+    o.initialize_reference()
+    done = 0
+    for istep, name in ((0, "n00000_t00.000000.dat"), (8, "n00008_t00.008000.dat"), (15, "n00015_t00.015000.dat")):
+        o.step(istep - done)
+        done = istep
+        a = read_table(d / name, 5)
+        assert a.shape[0] == 33 * 33
+        grid = a.reshape(33, 33, 5)
+        # periodic closure of file_access.cu:266-286
+        assert np.array_equal(grid[32, :32, 2:], grid[0, :32, 2:])
+        assert np.array_equal(grid[:32, 32, 2:], grid[:32, 0, 2:])
+        assert np.allclose(grid[:32, 0, 0], np.arange(32) * np.pi / 32)
+        for col, ring in ((2, "aomg"), (3, "aphi"), (4, "arho")):
+            ref = o.ktox_shear(o.buffer(ring))
+            assert rel_l2(grid[:32, :32, col], ref) < 1e-9, (name, ring)
+
+
+def test_spectral_dumps(run_dir):
+    d, _ = run_dir
+    o = oracle.Oracle(nx=32, ny=32, Lx=np.pi, Ly=np.pi, delt=1e-3, cfl_num=0.5, nu=1e-3, kappa=8e-5,
+                      sigma=1.0, g=1.0, rho0=1.0, rho0_prime=1.0, rho_eps1=1e-2, noise=1, linear=0)
+    o.initialize_reference()
+    o.step(1)
+    text = open(d / "befk_n00001_t00.001000.dat").read()
+    omg = o.buffer("aomg")
+    lines = [l for l in text.splitlines() if l and not l.startswith("/")]
+    re = np.array([[float(x) for x in l.split()] for l in lines[:omg.shape[0]]])
+    assert re.shape == omg.shape
+    assert rel_l2(re, omg.real) < 1e-9
+    o.step(7)
+    ks = read_table(d / "ksre_t00.008000.dat", 5)
+    nkx, nky = o.shape_k
+    assert ks.shape[0] == nkx * nky
+    assert np.all(np.diff(ks[::nky, 0]) > 0)   # rows in ascending kx (kx_index order)
+    assert rel_l2(np.sort(ks[:, 2]), np.sort(np.abs(o.buffer("aomg").real).ravel())) < 1e-9

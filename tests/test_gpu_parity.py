@@ -211,7 +211,14 @@ def test_cfl_halving(gpu, prec):
     assert so["halvings"] > 0
     assert sg["halvings"] == so["halvings"]
     assert sg["delt"] == so["delt"]
-    assert len(g.halvings()) >= 1
+    recs = g.halvings()
+    assert len(recs) >= 1
+    assert sum(r["count"] for r in recs) == sg["halvings"]
+    for r in recs:
+        # the per-condition counts replay the printed halvings of check_cfl
+        assert sum(r["counts"]) == r["count"]
+        assert r["delt_before"] / 2.0 ** r["count"] == r["delt"]
+        assert r["values"][2] == 1.0 and r["values"][3] == 1.0   # g/rho0, rho0_prime
     assert rel_l2(g.spectral(shrt.RHO), o.buffer("arho")) < TOLN[prec]
 
 
