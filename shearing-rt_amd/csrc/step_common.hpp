@@ -37,6 +37,11 @@ template <typename T, int N> constexpr int p1_points() { return TeamShape<N>::P;
 // Rows (teams) per P1 workgroup: 1 (one transform per workgroup) or kTile.
 template <typename T, int N> constexpr int p1_teams() { return 1; }
 template <typename T, int N> constexpr int p2_points() { return TeamShape<N>::P; }
+// P2 with three x-inverse transforms (three real fields live across the last)
+// instead of four (two live): one 2048-point transform fewer per column, at
+// the price of ~9 spilled registers at 2048 (34.4 against 36.8 us per launch at
+// 2048^2 FP32).
+template <typename T, int N> constexpr bool p2_three_pass() { return true; }
 // Columns (teams) per P2 workgroup: 1, 2 or kTile.
 template <typename T, int N> constexpr int p2_teams() { return 1; }
 template <typename T, int N> constexpr int p3_points() { return TeamShape<N>::P; }

@@ -318,22 +318,28 @@ k_p2(StepArgs<T> a) {
     double m0 = 0.0, m1 = 0.0;
 
     // Inverse x-transforms of G field pairs FA + i FB for this team's column.
-    // Nonlinear: (vx, dw/dx), (vy, dw/dy), (vx, dr/dx), (vy, dr/dy), so each
-    // velocity meets the derivative it multiplies and only the two brackets
-    // stay live between transforms; linear: (vx, vy) for the CFL maxima only.
+    // Four-pass form: (vx, dw/dx), (vy, dw/dy), (vx, dr/dx), (vy, dr/dy), so
+    // each velocity meets the derivative it multiplies and only the two
+    // brackets stay live between transforms.  Three-pass form (THREE): (vx,
+    // vy), (dw/dx, dw/dy), (dr/dx, dr/dy).  Linear: (vx, vy) for the CFL maxima
+    // only.
     // The fields of G: 0 vx = X(i kys phi), 1 vy = X(-i kx phi), 2 X(i kx w),
     // 3 X(i kys w), 4 X(i kx r), 5 X(i kys r).  The loop is not unrolled and
     // its lane index is made opaque per pass, so addresses are recomputed
     // rather than held in registers across transforms.
+    // THREE: the pairs (vx, vy), (dw/dx, dw/dy), (dr/dx, dr/dy), one transform
+    // fewer, with vx, vy and B_omega live across the last one.
+    constexpr bool THREE = p2_three_pass<T, NX>();
     T bw[P], br[P];
-    const int npass = a.ph.linear ? 1 : 4;
+    T vxs[THREE ? P : 1];
+    const int npass = a.ph.linear ? 1 : (THREE ? 3 : 4);
 #pragma unroll 1
     for (int pass = 0; pass < npass; ++pass) {
         int lane = threadIdx.x;
         asm volatile("" : "+v"(lane));
         const int tl = (lane - team * TT) & (TT - 1);
-        const int fa = a.ph.linear ? 0 : (pass & 1);
-        const int fb = a.ph.linear ? 1 : 2 + pass;
+        const int fa = a.ph.linear ? 0 : (THREE ? 2 * pass : (pass & 1));
+        const int fb = a.ph.linear ? 1 : (THREE ? 2 * pass + 1 : 2 + pass);
         const cplx<T>* GA = a.G + (long long)fa * g.I + (long long)tile * nk * 4;
         const cplx<T>* GB = a.G + (long long)fb * g.I + (long long)tile * nk * 4;
         // stage the two tiles as [field][c][kx], all loads first
@@ -380,6 +386,23 @@ k_p2(StepArgs<T> a) {
             for (int m = 0; m < P; ++m) {
                 m0 = fmax(m0, fabs(__dadd_rn((double)v[m].x, sy)));
                 m1 = fmax(m1, fabs((double)v[m].y));
+            }
+        } else if (THREE) {
+            // bw holds vx until B_omega exists, br holds vy until B_rho replaces it
+            if (pass == 0) {
+#pragma unroll
+                for (int m = 0; m < P; ++m) {
+                    m0 = fmax(m0, fabs(__dadd_rn((double)v[m].x, sy)));
+                    m1 = fmax(m1, fabs((double)v[m].y));
+                    vxs[m] = v[m].x;
+                    br[m] = v[m].y;
+                }
+            } else if (pass == 1) {
+#pragma unroll
+                for (int m = 0; m < P; ++m) bw[m] = vxs[m] * v[m].x + br[m] * v[m].y;
+            } else {
+#pragma unroll
+                for (int m = 0; m < P; ++m) br[m] = vxs[m] * v[m].x + br[m] * v[m].y;
             }
         } else if (pass == 0) {
 #pragma unroll
