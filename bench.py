@@ -123,6 +123,34 @@ def cpu_baseline(orc, n, params, ic, budget_s):
                       f"{threads} OpenMP threads, {dt:.2f} s"}
 
 
+def run_ensemble(shrt, args):
+    """Aggregate throughput of independent instances stepped by one captured
+    graph per batch (shrt_ensemble_*)."""
+    rng = np.random.default_rng(7)
+    members, points = [], 0
+    for part in args.ensemble.split(","):
+        n, cnt = (int(x) for x in part.split("x"))
+        for _ in range(cnt):
+            sigma = float(rng.uniform(0.0, 2.0))
+            g = shrt.Shrt(nx=n, ny=n, precision=shrt.FP32, graph_steps=0, Lx=2 * np.pi, Ly=2 * np.pi,
+                          delt=2e-4, cfl_num=0.3, nu=1e-4, kappa=1e-4, sigma=sigma, g=float(rng.uniform(0.5, 2)),
+                          rho0=1.0, rho0_prime=1.0, rho_eps1=1e-2, noise=0, linear=0)
+            g.initialize_seeded(seed=int(rng.integers(1 << 62)), kmin=2.0, kmax=max(2.0, n / 16), amp=1e-4, expo=-1.5)
+            members.append(g)
+            points += n * n
+    ens = shrt.Ensemble(members, steps_per_graph=args.graph_steps)
+    ens.step(args.graph_steps)   # warm-up and capture
+    steps = max(args.graph_steps, args.steps // args.graph_steps * args.graph_steps)
+    ms = ens.time_steps(steps)
+    out = {"metric": "ensemble grid-pt*steps/s", "value": points * steps / (ms * 1e-3),
+           "unit": "grid-pt*steps/s", "instances": len(members), "spec": args.ensemble, "steps": steps,
+           "ms_per_step": ms / steps, "dtype": "fp32", "graph_steps": args.graph_steps}
+    ens.close()
+    for g in members:
+        g.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -140,6 +168,9 @@ def main():
                                                        "CAL_WRITE_CSV", "CAL_BYTES"),
                     help="only turn rocprofv3 FETCH_SIZE/WRITE_SIZE counter files into HBM bytes per "
                          "dispatch (JSON on stdout) and exit")
+    ap.add_argument("--ensemble", default="", metavar="SPEC",
+                    help="only time an ensemble, e.g. 128x24,256x24,512x16 (grid x count; FP32, mixed "
+                         "shear rates and seeds, one captured graph per --graph-steps steps) and exit")
     args = ap.parse_args()
 
     if args.pmc_summary:
@@ -168,6 +199,10 @@ def main():
     if args.copy_calibrate:
         ms, gbs = copy_gbs(args.copy_calibrate, 5)
         print(json.dumps({"copy_bytes": args.copy_calibrate, "ms": ms, "GB/s": gbs}), flush=True)
+        return
+
+    if args.ensemble:
+        run_ensemble(shrt, args)
         return
 
     n = args.n

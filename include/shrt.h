@@ -186,6 +186,22 @@ shrt_status shrt_kernel_times(shrt_ctx* ctx, int nsteps, double* ms3);
  * reference point that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings. */
 shrt_status shrt_copy_bench(long long bytes, int reps, double* ms);
 
+/* Ensembles (new; no reference counterpart: the reference runs one instance
+ * per process).  Independent contexts -- any sizes, precisions, parameters and
+ * seeds -- are advanced together: each full batch of steps_per_graph steps (even)
+ * is one captured HIP graph holding every member's step chain, so small
+ * instances fill the GPU together.  Members keep their own state and API; a
+ * member must not be destroyed before its ensemble.  The results equal the
+ * members' individual runs bit for bit. */
+typedef struct shrt_ensemble shrt_ensemble;
+shrt_status shrt_ensemble_create(shrt_ctx* const* members, int n, int steps_per_graph, shrt_ensemble** out);
+shrt_status shrt_ensemble_destroy(shrt_ensemble* ens);
+/* nsteps steps of every member (full graphs, then single steps per member). */
+shrt_status shrt_ensemble_step(shrt_ensemble* ens, int nsteps);
+shrt_status shrt_ensemble_synchronize(shrt_ensemble* ens);
+/* Timed run of nsteps (a multiple of steps_per_graph) between HIP events. */
+shrt_status shrt_ensemble_time_steps(shrt_ensemble* ens, int nsteps, double* ms_total);
+
 /* Sizes derived by init_fft (fft.cu:91-96): out[0..7] = nkx, nky, nkxh, nkxh2,
  * nkxpad, ncy, nkx*nky, nkxh*ny. */
 shrt_status shrt_sizes(int nx, int ny, int* out8);

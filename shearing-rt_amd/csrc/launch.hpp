@@ -14,9 +14,13 @@ struct Launch {
     static bool supported(int n);
     static int team_threads(int n);
     static int p2_workgroups(int nx, int ny);   // the P2 grid (number of CFL partials)
-    static hipError_t p1(const StepArgs<T>& a, int grid, hipStream_t s);
-    static hipError_t p2(const StepArgs<T>& a, int grid, hipStream_t s);
-    static hipError_t p3(const StepArgs<T>& a, int grid, hipStream_t s);
+    // The step kernels.  dargs == nullptr: one instance, arguments `geom` passed
+    // by value.  Otherwise `count` instances of one shape whose argument blocks
+    // are in device memory at `dargs` (instance i runs as blockIdx.y == i);
+    // `geom` is any one of them (host copy, for the grid).
+    static hipError_t p1(const StepArgs<T>& geom, const StepArgs<T>* dargs, int count, hipStream_t s);
+    static hipError_t p2(const StepArgs<T>& geom, const StepArgs<T>* dargs, int count, hipStream_t s);
+    static hipError_t p3(const StepArgs<T>& geom, const StepArgs<T>* dargs, int count, hipStream_t s);
     static hipError_t xfwd(const AuxArgs<T>& a, hipStream_t s);
     static hipError_t yfwd(const AuxArgs<T>& a, hipStream_t s);
     static hipError_t yinv(const AuxArgs<T>& a, hipStream_t s);

@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <map>
 #include <vector>
 
 #include "../../include/shrt.h"
@@ -154,12 +155,12 @@ AuxArgs<T> aux_args(shrt_ctx* c, const void* in, void* out, const double* s, con
 
 template <typename T>
 hipError_t enqueue_step_t(shrt_ctx* c, int in_par, int store_phi) {
-    StepArgs<T> a = step_args<T>(c, in_par, store_phi);
-    hipError_t e = Launch<T>::p1(a, c->g.ntile_x * (c->ph.linear ? 1 : 3), c->stream);
+    const StepArgs<T> a = step_args<T>(c, in_par, store_phi);
+    hipError_t e = Launch<T>::p1(a, nullptr, 1, c->stream);
     if (e != hipSuccess) return e;
-    e = Launch<T>::p2(a, c->g.ntile_y, c->stream);
+    e = Launch<T>::p2(a, nullptr, 1, c->stream);
     if (e != hipSuccess) return e;
-    return Launch<T>::p3(a, 2 * c->g.ntile_x, c->stream);
+    return Launch<T>::p3(a, nullptr, 1, c->stream);
 }
 
 hipError_t enqueue_step(shrt_ctx* c, int in_par, int store_phi) {
@@ -637,16 +638,17 @@ shrt_status shrt_kernel_times(shrt_ctx* c, int nsteps, double* ms3) {
         hipEvent_t* e = &ev[(size_t)k * 4];
         const int par = c->cur;
         auto launch = [&](int which) -> hipError_t {
+            const int sp = k == nsteps - 1;
             if (c->prec == 4) {
-                StepArgs<float> a = step_args<float>(c, par, k == nsteps - 1);
-                return which == 0 ? Launch<float>::p1(a, c->g.ntile_x * (c->ph.linear ? 1 : 3), c->stream)
-                     : which == 1 ? Launch<float>::p2(a, c->g.ntile_y, c->stream)
-                                  : Launch<float>::p3(a, 2 * c->g.ntile_x, c->stream);
+                const StepArgs<float> a = step_args<float>(c, par, sp);
+                return which == 0 ? Launch<float>::p1(a, nullptr, 1, c->stream)
+                     : which == 1 ? Launch<float>::p2(a, nullptr, 1, c->stream)
+                                  : Launch<float>::p3(a, nullptr, 1, c->stream);
             }
-            StepArgs<double> a = step_args<double>(c, par, k == nsteps - 1);
-            return which == 0 ? Launch<double>::p1(a, c->g.ntile_x * (c->ph.linear ? 1 : 3), c->stream)
-                 : which == 1 ? Launch<double>::p2(a, c->g.ntile_y, c->stream)
-                              : Launch<double>::p3(a, 2 * c->g.ntile_x, c->stream);
+            const StepArgs<double> a = step_args<double>(c, par, sp);
+            return which == 0 ? Launch<double>::p1(a, nullptr, 1, c->stream)
+                 : which == 1 ? Launch<double>::p2(a, nullptr, 1, c->stream)
+                              : Launch<double>::p3(a, nullptr, 1, c->stream);
         };
         for (int w = 0; w < 3 && err == hipSuccess; ++w) {
             err = hipEventRecord(e[w], c->stream);
@@ -918,6 +920,215 @@ shrt_status shrt_get_halvings(shrt_ctx* c, shrt_halving* out, int cap, int* coun
     }
     *count = n;
     return SHRT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Ensembles: independent instances advanced by one captured graph
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+struct shrt_ensemble {
+    std::vector<shrt_ctx*> m;
+    std::vector<std::vector<int>> groups;                 // members of one precision and shape
+    int K = 0;                                            // steps per graph (even)
+    hipStream_t stream = nullptr;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> joins;
+    struct Graph {
+        hipGraphExec_t exec = nullptr;
+        std::vector<void*> bufs;                          // the batches' argument blocks
+    };
+    std::map<std::vector<int>, Graph> graphs;             // by the members' step parities
+};
+
+namespace {
+
+// The argument blocks of one group for every (step parity, store_phi): member
+// i at block [(2*k1 + sp) * n + i] steps from parity par_i ^ k1.
+template <typename T>
+hipError_t group_args(shrt_ensemble* e, const std::vector<int>& grp, const std::vector<int>& par, void** out) {
+    const size_t n = grp.size();
+    std::vector<StepArgs<T>> h(4 * n);
+    for (int k1 = 0; k1 < 2; ++k1)
+        for (int sp = 0; sp < 2; ++sp)
+            for (size_t i = 0; i < n; ++i)
+                h[(2 * k1 + sp) * n + i] = step_args<T>(e->m[grp[i]], par[grp[i]] ^ k1, sp);
+    hipError_t err = hipMalloc(out, sizeof(StepArgs<T>) * h.size());
+    if (err != hipSuccess) return err;
+    return hipMemcpy(*out, h.data(), sizeof(StepArgs<T>) * h.size(), hipMemcpyHostToDevice);
+}
+
+template <typename T>
+hipError_t group_step(shrt_ensemble* e, const std::vector<int>& grp, const void* args, int k, hipStream_t s) {
+    const size_t n = grp.size();
+    const int sp = k == e->K - 1;
+    const StepArgs<T>* d = static_cast<const StepArgs<T>*>(args) + (2 * (k & 1) + sp) * n;
+    if (n == 1) d = nullptr;   // a group of one takes its arguments by value
+    const StepArgs<T> geom = step_args<T>(e->m[grp[0]], e->m[grp[0]]->cur ^ (k & 1), sp);
+    hipError_t err = Launch<T>::p1(geom, d, (int)n, s);
+    if (err == hipSuccess) err = Launch<T>::p2(geom, d, (int)n, s);
+    if (err == hipSuccess) err = Launch<T>::p3(geom, d, (int)n, s);
+    return err;
+}
+
+// One graph holding, per group of equal precision and shape, a chain of K
+// batched steps (each kernel launched once for all the group's members, the
+// member as blockIdx.y); the chains fork from and join into the ensemble's
+// stream, so groups run concurrently.
+shrt_status ensemble_graph(shrt_ensemble* e, const std::vector<int>& par, hipGraphExec_t* out) {
+    auto it = e->graphs.find(par);
+    if (it != e->graphs.end()) { *out = it->second.exec; return SHRT_OK; }
+    shrt_ensemble::Graph gr;
+    for (const auto& grp : e->groups) {
+        void* buf = nullptr;
+        const hipError_t err = e->m[grp[0]]->prec == 4 ? group_args<float>(e, grp, par, &buf)
+                                                       : group_args<double>(e, grp, par, &buf);
+        if (buf) gr.bufs.push_back(buf);
+        if (err != hipSuccess) {
+            for (void* b : gr.bufs) (void)hipFree(b);
+            return fail(SHRT_ERR_ALLOC, std::string("ensemble arguments: ") + hipGetErrorString(err));
+        }
+    }
+    hipGraph_t graph = nullptr;
+    HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t err = hipEventRecord(e->fork, e->stream);
+    for (size_t gi = 0; gi < e->groups.size() && err == hipSuccess; ++gi) {
+        const auto& grp = e->groups[gi];
+        hipStream_t s = e->m[grp[0]]->stream;
+        err = hipStreamWaitEvent(s, e->fork, 0);
+        for (int k = 0; k < e->K && err == hipSuccess; ++k)
+            err = e->m[grp[0]]->prec == 4 ? group_step<float>(e, grp, gr.bufs[gi], k, s)
+                                          : group_step<double>(e, grp, gr.bufs[gi], k, s);
+        if (err == hipSuccess) err = hipEventRecord(e->joins[gi], s);
+        if (err == hipSuccess) err = hipStreamWaitEvent(e->stream, e->joins[gi], 0);
+    }
+    hipError_t e2 = hipStreamEndCapture(e->stream, &graph);
+    hipError_t e3 = hipSuccess;
+    if (err == hipSuccess && e2 == hipSuccess) e3 = hipGraphInstantiate(&gr.exec, graph, nullptr, nullptr, 0);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (err != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+        for (void* b : gr.bufs) (void)hipFree(b);
+        const hipError_t bad = err != hipSuccess ? err : (e2 != hipSuccess ? e2 : e3);
+        return fail(SHRT_ERR_DEVICE, std::string("ensemble graph: ") + hipGetErrorString(bad));
+    }
+    e->graphs[par] = gr;
+    *out = gr.exec;
+    return SHRT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+shrt_status shrt_ensemble_create(shrt_ctx* const* members, int n, int steps_per_graph, shrt_ensemble** out) {
+    if (!members || !out || n < 1) return fail(SHRT_ERR_INVALID, "null argument or empty ensemble");
+    if (steps_per_graph < 2 || (steps_per_graph & 1)) return fail(SHRT_ERR_INVALID, "steps_per_graph must be even and >= 2");
+    for (int i = 0; i < n; ++i) {
+        if (!members[i]) return fail(SHRT_ERR_INVALID, "null member");
+        for (int j = 0; j < i; ++j)
+            if (members[j] == members[i]) return fail(SHRT_ERR_INVALID, "a context appears twice");
+    }
+    auto* e = new shrt_ensemble;
+    e->m.assign(members, members + n);
+    e->K = steps_per_graph;
+    for (int i = 0; i < n; ++i) {   // group by precision and shape
+        const shrt_ctx* c = members[i];
+        size_t gi = 0;
+        for (; gi < e->groups.size(); ++gi) {
+            const shrt_ctx* r = members[e->groups[gi][0]];
+            if (r->prec == c->prec && r->g.nx == c->g.nx && r->g.ny == c->g.ny && r->ph.linear == c->ph.linear) break;
+        }
+        if (gi == e->groups.size()) e->groups.emplace_back();
+        e->groups[gi].push_back(i);
+    }
+    e->joins.assign(e->groups.size(), nullptr);
+    bool ok = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&e->fork, hipEventDisableTiming) == hipSuccess;
+    for (size_t i = 0; ok && i < e->joins.size(); ++i)
+        ok = hipEventCreateWithFlags(&e->joins[i], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        shrt_ensemble_destroy(e);
+        return fail(SHRT_ERR_DEVICE, "ensemble stream or event creation failed");
+    }
+    *out = e;
+    return SHRT_OK;
+}
+
+shrt_status shrt_ensemble_destroy(shrt_ensemble* e) {
+    if (!e) return SHRT_OK;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    for (auto& kv : e->graphs) {
+        (void)hipGraphExecDestroy(kv.second.exec);
+        for (void* b : kv.second.bufs) (void)hipFree(b);
+    }
+    for (hipEvent_t ev : e->joins)
+        if (ev) (void)hipEventDestroy(ev);
+    if (e->fork) (void)hipEventDestroy(e->fork);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+    return SHRT_OK;
+}
+
+shrt_status shrt_ensemble_step(shrt_ensemble* e, int nsteps) {
+    if (!e || nsteps < 0) return fail(SHRT_ERR_INVALID, "shrt_ensemble_step: bad arguments");
+    const size_t n = e->m.size();
+    int full = nsteps / e->K;
+    if (full > 0) {
+        std::vector<int> par(n);
+        for (size_t i = 0; i < n; ++i) par[i] = e->m[i]->cur;
+        hipGraphExec_t ex = nullptr;
+        shrt_status st = ensemble_graph(e, par, &ex);
+        if (st != SHRT_OK) return st;
+        // the graph runs after each member's earlier work, and before its later work
+        for (size_t i = 0; i < n; ++i) {
+            HIP_TRY(hipEventRecord(e->fork, e->m[i]->stream));
+            HIP_TRY(hipStreamWaitEvent(e->stream, e->fork, 0));
+        }
+        for (int b = 0; b < full; ++b) HIP_TRY(hipGraphLaunch(ex, e->stream));   // K even: parities unchanged
+        HIP_TRY(hipEventRecord(e->fork, e->stream));
+        for (size_t i = 0; i < n; ++i) HIP_TRY(hipStreamWaitEvent(e->m[i]->stream, e->fork, 0));
+    }
+    const int rest = nsteps - full * e->K;
+    for (size_t i = 0; i < n && rest > 0; ++i) {
+        shrt_status st = shrt_step_async(e->m[i], rest);
+        if (st != SHRT_OK) return st;
+    }
+    return SHRT_OK;
+}
+
+shrt_status shrt_ensemble_synchronize(shrt_ensemble* e) {
+    if (!e) return fail(SHRT_ERR_INVALID, "null ensemble");
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    for (shrt_ctx* c : e->m) {
+        shrt_status st = shrt_synchronize(c);
+        if (st != SHRT_OK) return st;
+    }
+    return SHRT_OK;
+}
+
+shrt_status shrt_ensemble_time_steps(shrt_ensemble* e, int nsteps, double* ms_total) {
+    if (!e || !ms_total || nsteps < e->K || nsteps % e->K) return fail(SHRT_ERR_INVALID, "bad arguments");
+    std::vector<int> par(e->m.size());
+    for (size_t i = 0; i < e->m.size(); ++i) par[i] = e->m[i]->cur;
+    hipGraphExec_t ex = nullptr;
+    shrt_status st = ensemble_graph(e, par, &ex);   // built outside the timed region
+    if (st != SHRT_OK) return st;
+    st = shrt_ensemble_synchronize(e);
+    if (st != SHRT_OK) return st;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, e->stream));
+    st = shrt_ensemble_step(e, nsteps);
+    HIP_TRY(hipEventRecord(e1, e->stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st != SHRT_OK) return st;
+    *ms_total = ms;
+    return shrt_ensemble_synchronize(e);
 }
 
 }  // extern "C"

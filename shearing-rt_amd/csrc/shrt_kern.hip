@@ -68,40 +68,42 @@ int Launch<R>::team_threads(int n) {
 }
 
 template <>
-hipError_t Launch<R>::p1(const StepArgs<R>& a, int grid, hipStream_t s) {
-    (void)grid;
+hipError_t Launch<R>::p1(const StepArgs<R>& a, const StepArgs<R>* dargs, int count, hipStream_t s) {
     bool ok = dispatch(a.g.ny, [&](auto nc) {
         constexpr int N = decltype(nc)::value;
         if constexpr (p1_teams<R, N>() == 1) {
             const int items = (a.g.nkxh - 1) * (a.ph.linear ? 2 : 6) + (a.ph.linear ? 1 : 3);
-            hipLaunchKernelGGL((k_p1_row<R, N>), dim3(items), dim3(TeamShape<N>::TT), 0, s, a);
+            if (dargs) hipLaunchKernelGGL((k_p1_row<R, N, true>), dim3(items, count), dim3(TeamShape<N>::TT), 0, s, a, dargs);
+            else hipLaunchKernelGGL((k_p1_row<R, N, false>), dim3(items), dim3(TeamShape<N>::TT), 0, s, a, dargs);
         } else {
             const P1Grid pg(a.g.nkxh, a.ph.linear);
-            hipLaunchKernelGGL((k_p1<R, N>), dim3(pg.grid), dim3(kTile * TeamShape<N>::TT), 0, s, a);
+            if (dargs) hipLaunchKernelGGL((k_p1<R, N, true>), dim3(pg.grid, count), dim3(kTile * TeamShape<N>::TT), 0, s, a, dargs);
+            else hipLaunchKernelGGL((k_p1<R, N, false>), dim3(pg.grid), dim3(kTile * TeamShape<N>::TT), 0, s, a, dargs);
         }
     });
     return ok ? hipGetLastError() : unsupported();
 }
 
 template <>
-hipError_t Launch<R>::p2(const StepArgs<R>& a, int grid, hipStream_t s) {
-    (void)grid;
+hipError_t Launch<R>::p2(const StepArgs<R>& a, const StepArgs<R>* dargs, int count, hipStream_t s) {
     bool ok = dispatch(a.g.nx, [&](auto nc) {
         constexpr int N = decltype(nc)::value;
         constexpr int KT = p2_teams<R, N>();
-        hipLaunchKernelGGL((k_p2<R, N>), dim3(a.g.ny / KT), dim3(KT * Team<N, p2_points<R, N>()>::TT), 0, s, a);
+        const dim3 block(KT * Team<N, p2_points<R, N>()>::TT);
+        if (dargs) hipLaunchKernelGGL((k_p2<R, N, true>), dim3(a.g.ny / KT, count), block, 0, s, a, dargs);
+        else hipLaunchKernelGGL((k_p2<R, N, false>), dim3(a.g.ny / KT), block, 0, s, a, dargs);
     });
     return ok ? hipGetLastError() : unsupported();
 }
 
 template <>
-hipError_t Launch<R>::p3(const StepArgs<R>& a, int grid, hipStream_t s) {
-    (void)grid;
+hipError_t Launch<R>::p3(const StepArgs<R>& a, const StepArgs<R>* dargs, int count, hipStream_t s) {
     bool ok = dispatch(a.g.ny, [&](auto nc) {
         constexpr int N = decltype(nc)::value;
         constexpr int KT = p3_teams<R, N>();
         const int groups = (a.g.nkxh + KT - 1) / KT;
-        hipLaunchKernelGGL((k_p3<R, N>), dim3(2 * groups), dim3(KT * TeamShape<N>::TT), 0, s, a);
+        if (dargs) hipLaunchKernelGGL((k_p3<R, N, true>), dim3(2 * groups, count), dim3(KT * TeamShape<N>::TT), 0, s, a, dargs);
+        else hipLaunchKernelGGL((k_p3<R, N, false>), dim3(2 * groups), dim3(KT * TeamShape<N>::TT), 0, s, a, dargs);
     });
     return ok ? hipGetLastError() : unsupported();
 }

@@ -25,6 +25,14 @@
 
 namespace shrt {
 
+// The argument block of a step kernel: the kernel argument itself, or, for a
+// batch of instances of one shape, entry blockIdx.y of a device array.
+template <bool BATCH, typename T>
+__device__ __forceinline__ const StepArgs<T>& step_args_of(const StepArgs<T>& a1, const StepArgs<T>* args) {
+    if constexpr (BATCH) return args[blockIdx.y];
+    else return a1;
+}
+
 // ---------------------------------------------------------------------------
 // P1
 // ---------------------------------------------------------------------------
@@ -160,9 +168,10 @@ __device__ __forceinline__ void p1_store_phi(const StepArgs<T>& a, int row, int 
 // Items beyond: the leftover rows, one (row, output) transform per team,
 // written by the team directly; the first of them also writes the vanishing
 // outputs of row 0.
-template <typename T, int NY>
+template <typename T, int NY, bool BATCH>
 __global__ void __launch_bounds__(kTile * TeamShape<NY>::TT, (waves_per_eu<T, NY>()))
-k_p1(StepArgs<T> a) {
+k_p1(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
+    const StepArgs<T>& a = step_args_of<BATCH>(a1, args);
     using S = TeamShape<NY>;
     constexpr int P = S::P, TT = S::TT, L = lds_len<NY>();
     constexpr int LP = stage_stride<NY>();
@@ -241,9 +250,10 @@ k_p1(StepArgs<T> a) {
 // ones.  Each team writes its G position directly, in runs of 4 (32 bytes) per
 // t; the outputs of neighbouring rows are dispatched together to one XCD, whose
 // L2 merges the runs into whole lines.
-template <typename T, int NY>
+template <typename T, int NY, bool BATCH>
 __global__ void __launch_bounds__(TeamShape<NY>::TT, (waves_per_eu<T, NY, TeamShape<NY>::TT, 1>()))
-k_p1_row(StepArgs<T> a) {
+k_p1_row(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
+    const StepArgs<T>& a = step_args_of<BATCH>(a1, args);
     using S = TeamShape<NY>;
     constexpr int P = S::P, TT = S::TT, L = lds_len<NY>();
     __shared__ cplx<T> sm[L];
@@ -286,10 +296,11 @@ k_p1_row(StepArgs<T> a) {
 // Workgroups of P2: KT = p2_teams columns each (a quarter, half or all of a
 // G tile of kTile columns); the workgroups of one tile run on one XCD so the
 // tile's lines are fetched once into its L2.
-template <typename T, int NX>
+template <typename T, int NX, bool BATCH>
 __global__ void __launch_bounds__((p2_teams<T, NX>() * Team<NX, p2_points<T, NX>()>::TT),
                                   (waves_per_eu<T, NX, Team<NX, p2_points<T, NX>()>::TT, p2_teams<T, NX>()>()))
-k_p2(StepArgs<T> a) {
+k_p2(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
+    const StepArgs<T>& a = step_args_of<BATCH>(a1, args);
     using S = Team<NX, p2_points<T, NX>()>;
     constexpr int KT = p2_teams<T, NX>();
     constexpr int SUB = kTile / KT;                    // workgroups per G tile
@@ -595,10 +606,11 @@ __device__ void p3_bookkeeping(const StepArgs<T>& a, double dt, int h, int which
 // Workgroups of P3: (group of KT row pairs, field), KT = p3_teams.  One row
 // pair per workgroup (KT = 1) gives eight small workgroups per CU instead of
 // two large ones, so their load, transform and update phases overlap.
-template <typename T, int NY>
+template <typename T, int NY, bool BATCH>
 __global__ void __launch_bounds__((p3_teams<T, NY>() * TeamShape<NY>::TT),
                                   (waves_per_eu<T, NY, TeamShape<NY>::TT, p3_teams<T, NY>()>()))
-k_p3(StepArgs<T> a) {
+k_p3(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
+    const StepArgs<T>& a = step_args_of<BATCH>(a1, args);
     using S = TeamShape<NY>;
     constexpr int KT = p3_teams<T, NY>();
     constexpr int P = S::P, TT = S::TT, L = lds_len<NY>();

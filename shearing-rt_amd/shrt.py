@@ -83,6 +83,12 @@ _SIGS = {
     "shrt_xtok": ([_VP, _VP, _VP], ctypes.c_int),
     "shrt_ktox": ([_VP, _VP, _VP], ctypes.c_int),
     "shrt_ktox_shear": ([_VP, _VP, _VP], ctypes.c_int),
+    "shrt_ensemble_create": ([ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int,
+                              ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
+    "shrt_ensemble_destroy": ([_VP], ctypes.c_int),
+    "shrt_ensemble_step": ([_VP, ctypes.c_int], ctypes.c_int),
+    "shrt_ensemble_synchronize": ([_VP], ctypes.c_int),
+    "shrt_ensemble_time_steps": ([_VP, ctypes.c_int, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "shrt_shearing_field": ([_VP, _VP, _VP, _VP], ctypes.c_int),
     "shrt_get_time": ([_VP, ctypes.POINTER(Time)], ctypes.c_int),
     "shrt_set_time": ([_VP, ctypes.c_int64, ctypes.c_double, ctypes.c_double], ctypes.c_int),
@@ -283,6 +289,38 @@ class Shrt:
         return [dict(istep=arr[i].istep, time=arr[i].time, value=arr[i].value, delt=arr[i].delt,
                      which=arr[i].which, count=arr[i].count, counts=list(arr[i].counts),
                      values=list(arr[i].values), delt_before=arr[i].delt_before) for i in range(n.value)]
+
+
+class Ensemble:
+    """Independent contexts advanced together, one captured graph per batch of
+    `steps_per_graph` steps (shrt_ensemble_*)."""
+
+    def __init__(self, members, steps_per_graph=8):
+        self.members = list(members)
+        arr = (ctypes.c_void_p * len(self.members))(*[m.h.value for m in self.members])
+        h = ctypes.c_void_p()
+        _check(lib().shrt_ensemble_create(arr, len(self.members), int(steps_per_graph), ctypes.byref(h)))
+        self.h = h
+
+    def step(self, n):
+        _check(lib().shrt_ensemble_step(self.h, int(n)))
+        _check(lib().shrt_ensemble_synchronize(self.h))
+
+    def time_steps(self, n):
+        ms = ctypes.c_double()
+        _check(lib().shrt_ensemble_time_steps(self.h, int(n), ctypes.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().shrt_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def header_symbols():
