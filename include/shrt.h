@@ -186,6 +186,22 @@ shrt_status shrt_kernel_times(shrt_ctx* ctx, int nsteps, double* ms3);
  * reference point that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings. */
 shrt_status shrt_copy_bench(long long bytes, int reps, double* ms);
 
+/* Autotuning (new; no reference counterpart).  Each pass of the step has two
+ * kernel variants: P1 one (row pair, output) per workgroup (0) or a tile of four
+ * row pairs (1); P2 three x-inverse transforms per column (0) or four (1); P3
+ * one row pair per workgroup (0) or four (1).  shrt_autotune times both
+ * variants of each pass on a scratch instance of the context's shape and
+ * parameters (reps launches each) and keeps the faster.  The variants agree
+ * to rounding (the P2 variants pair the fields differently in the complex
+ * transforms; all are compiled separately).  ms[] = average ms per launch of
+ * p1 row, p1 tile, p2 three, p2 four, p3 pair, p3 tile. */
+typedef struct {
+    int p1, p2, p3;
+    double ms[6];
+} shrt_tuning;
+shrt_status shrt_autotune(shrt_ctx* ctx, int reps, shrt_tuning* out);
+shrt_status shrt_set_variants(shrt_ctx* ctx, int p1, int p2, int p3);
+
 /* Ensembles (new; no reference counterpart: the reference runs one instance
  * per process).  Independent contexts -- any sizes, precisions, parameters and
  * seeds -- are advanced together: each full batch of steps_per_graph steps (even)

@@ -18,9 +18,10 @@ struct Launch {
     // by value.  Otherwise `count` instances of one shape whose argument blocks
     // are in device memory at `dargs` (instance i runs as blockIdx.y == i);
     // `geom` is any one of them (host copy, for the grid).
-    static hipError_t p1(const StepArgs<T>& geom, const StepArgs<T>* dargs, int count, hipStream_t s);
-    static hipError_t p2(const StepArgs<T>& geom, const StepArgs<T>* dargs, int count, hipStream_t s);
-    static hipError_t p3(const StepArgs<T>& geom, const StepArgs<T>* dargs, int count, hipStream_t s);
+    // `variant` is that pass's member of Variants.
+    static hipError_t p1(const StepArgs<T>& geom, const StepArgs<T>* dargs, int count, int variant, hipStream_t s);
+    static hipError_t p2(const StepArgs<T>& geom, const StepArgs<T>* dargs, int count, int variant, hipStream_t s);
+    static hipError_t p3(const StepArgs<T>& geom, const StepArgs<T>* dargs, int count, int variant, hipStream_t s);
     static hipError_t xfwd(const AuxArgs<T>& a, hipStream_t s);
     static hipError_t yfwd(const AuxArgs<T>& a, hipStream_t s);
     static hipError_t yinv(const AuxArgs<T>& a, hipStream_t s);

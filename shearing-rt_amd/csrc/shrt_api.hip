@@ -8,6 +8,7 @@
 // needs the host.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -67,6 +68,7 @@ struct shrt_ctx {
     int* jump_tmp = nullptr;  // [nkx]
     HalvingRec* hlog = nullptr;
     double* cflp = nullptr;   // [P2 workgroups][2] CFL partial maxima of the x pass
+    Variants var;             // kernel variant of each pass (shrt_autotune)
     double* diag_partial = nullptr;
     double* diag_out = nullptr;
     int cur = 0;
@@ -156,11 +158,11 @@ AuxArgs<T> aux_args(shrt_ctx* c, const void* in, void* out, const double* s, con
 template <typename T>
 hipError_t enqueue_step_t(shrt_ctx* c, int in_par, int store_phi) {
     const StepArgs<T> a = step_args<T>(c, in_par, store_phi);
-    hipError_t e = Launch<T>::p1(a, nullptr, 1, c->stream);
+    hipError_t e = Launch<T>::p1(a, nullptr, 1, c->var.p1, c->stream);
     if (e != hipSuccess) return e;
-    e = Launch<T>::p2(a, nullptr, 1, c->stream);
+    e = Launch<T>::p2(a, nullptr, 1, c->var.p2, c->stream);
     if (e != hipSuccess) return e;
-    return Launch<T>::p3(a, nullptr, 1, c->stream);
+    return Launch<T>::p3(a, nullptr, 1, c->var.p3, c->stream);
 }
 
 hipError_t enqueue_step(shrt_ctx* c, int in_par, int store_phi) {
@@ -629,6 +631,93 @@ shrt_status shrt_copy_bench(long long bytes, int reps, double* ms) {
     return SHRT_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Average duration (ms) of `reps` back-to-back launches of one pass's variant on
+// context `t`, from the step state it holds (the passes are idempotent for a
+// fixed input state).
+template <typename T>
+shrt_status time_pass(shrt_ctx* t, int pass, int variant, int reps, double* ms) {
+    const StepArgs<T> a = step_args<T>(t, t->cur, 0);
+    auto launch = [&]() {
+        return pass == 0 ? Launch<T>::p1(a, nullptr, 1, variant, t->stream)
+             : pass == 1 ? Launch<T>::p2(a, nullptr, 1, variant, t->stream)
+                         : Launch<T>::p3(a, nullptr, 1, variant, t->stream);
+    };
+    HIP_TRY(launch());   // warm
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, t->stream));
+    hipError_t err = hipSuccess;
+    for (int r = 0; r < reps && err == hipSuccess; ++r) err = launch();
+    HIP_TRY(hipEventRecord(e1, t->stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    float f = 0.f;
+    HIP_TRY(hipEventElapsedTime(&f, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIP_TRY(err);
+    *ms = f / reps;
+    return SHRT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+shrt_status shrt_autotune(shrt_ctx* c, int reps, shrt_tuning* out) {
+    if (!c || reps < 1) return fail(SHRT_ERR_INVALID, "shrt_autotune: bad arguments");
+    // a scratch instance of the same shape and parameters, a few steps into a seeded run
+    shrt_ctx* t = nullptr;
+    shrt_params p = c->p;
+    p.graph_steps = 0;
+    shrt_status st = shrt_create(&p, &t);
+    if (st != SHRT_OK) return st;
+    shrt_seeded_ic ic{12345u, 2.0, std::max(2.0, c->g.ny / 16.0), 1e-4, -1.5};
+    st = shrt_initialize_seeded(t, &ic);
+    if (st == SHRT_OK) st = shrt_step(t, 4);
+    static const int nvar[3] = {2, 2, 2};
+    double ms[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+    for (int pass = 0; pass < 3 && st == SHRT_OK; ++pass)
+        for (int v = 0; v < nvar[pass] && st == SHRT_OK; ++v)
+            st = c->prec == 4 ? time_pass<float>(t, pass, v, reps, &ms[pass][v])
+                              : time_pass<double>(t, pass, v, reps, &ms[pass][v]);
+    (void)shrt_destroy(t);
+    if (st != SHRT_OK) return st;
+    c->var.p1 = ms[0][1] < ms[0][0] ? kP1Tile : kP1Row;
+    c->var.p2 = ms[1][1] < ms[1][0] ? kP2Four : kP2Three;
+    c->var.p3 = ms[2][1] < ms[2][0] ? kP3Tile : kP3Pair;
+    for (int i = 0; i < 2; ++i)   // captured graphs hold the old launches
+        if (c->gexec[i]) {
+            (void)hipGraphExecDestroy(c->gexec[i]);
+            c->gexec[i] = nullptr;
+        }
+    if (out) {
+        out->p1 = c->var.p1;
+        out->p2 = c->var.p2;
+        out->p3 = c->var.p3;
+        for (int pass = 0; pass < 3; ++pass)
+            for (int v = 0; v < 2; ++v) out->ms[2 * pass + v] = ms[pass][v];
+    }
+    return SHRT_OK;
+}
+
+shrt_status shrt_set_variants(shrt_ctx* c, int p1, int p2, int p3) {
+    if (!c || p1 < 0 || p1 > 1 || p2 < 0 || p2 > 1 || p3 < 0 || p3 > 1) return fail(SHRT_ERR_INVALID, "bad variant");
+    c->var.p1 = p1;
+    c->var.p2 = p2;
+    c->var.p3 = p3;
+    for (int i = 0; i < 2; ++i)
+        if (c->gexec[i]) {
+            (void)hipGraphExecDestroy(c->gexec[i]);
+            c->gexec[i] = nullptr;
+        }
+    return SHRT_OK;
+}
+
 shrt_status shrt_kernel_times(shrt_ctx* c, int nsteps, double* ms3) {
     if (!c || !ms3 || nsteps <= 0) return fail(SHRT_ERR_INVALID, "bad arguments");
     std::vector<hipEvent_t> ev((size_t)nsteps * 4);
@@ -641,14 +730,14 @@ shrt_status shrt_kernel_times(shrt_ctx* c, int nsteps, double* ms3) {
             const int sp = k == nsteps - 1;
             if (c->prec == 4) {
                 const StepArgs<float> a = step_args<float>(c, par, sp);
-                return which == 0 ? Launch<float>::p1(a, nullptr, 1, c->stream)
-                     : which == 1 ? Launch<float>::p2(a, nullptr, 1, c->stream)
-                                  : Launch<float>::p3(a, nullptr, 1, c->stream);
+                return which == 0 ? Launch<float>::p1(a, nullptr, 1, c->var.p1, c->stream)
+                     : which == 1 ? Launch<float>::p2(a, nullptr, 1, c->var.p2, c->stream)
+                                  : Launch<float>::p3(a, nullptr, 1, c->var.p3, c->stream);
             }
             const StepArgs<double> a = step_args<double>(c, par, sp);
-            return which == 0 ? Launch<double>::p1(a, nullptr, 1, c->stream)
-                 : which == 1 ? Launch<double>::p2(a, nullptr, 1, c->stream)
-                              : Launch<double>::p3(a, nullptr, 1, c->stream);
+            return which == 0 ? Launch<double>::p1(a, nullptr, 1, c->var.p1, c->stream)
+                 : which == 1 ? Launch<double>::p2(a, nullptr, 1, c->var.p2, c->stream)
+                              : Launch<double>::p3(a, nullptr, 1, c->var.p3, c->stream);
         };
         for (int w = 0; w < 3 && err == hipSuccess; ++w) {
             err = hipEventRecord(e[w], c->stream);
@@ -965,9 +1054,10 @@ hipError_t group_step(shrt_ensemble* e, const std::vector<int>& grp, const void*
     const StepArgs<T>* d = static_cast<const StepArgs<T>*>(args) + (2 * (k & 1) + sp) * n;
     if (n == 1) d = nullptr;   // a group of one takes its arguments by value
     const StepArgs<T> geom = step_args<T>(e->m[grp[0]], e->m[grp[0]]->cur ^ (k & 1), sp);
-    hipError_t err = Launch<T>::p1(geom, d, (int)n, s);
-    if (err == hipSuccess) err = Launch<T>::p2(geom, d, (int)n, s);
-    if (err == hipSuccess) err = Launch<T>::p3(geom, d, (int)n, s);
+    const Variants& v = e->m[grp[0]]->var;   // the group runs its first member's variants
+    hipError_t err = Launch<T>::p1(geom, d, (int)n, v.p1, s);
+    if (err == hipSuccess) err = Launch<T>::p2(geom, d, (int)n, v.p2, s);
+    if (err == hipSuccess) err = Launch<T>::p3(geom, d, (int)n, v.p3, s);
     return err;
 }
 

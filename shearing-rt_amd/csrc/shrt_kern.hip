@@ -68,42 +68,52 @@ int Launch<R>::team_threads(int n) {
 }
 
 template <>
-hipError_t Launch<R>::p1(const StepArgs<R>& a, const StepArgs<R>* dargs, int count, hipStream_t s) {
+hipError_t Launch<R>::p1(const StepArgs<R>& a, const StepArgs<R>* dargs, int count, int variant, hipStream_t s) {
     bool ok = dispatch(a.g.ny, [&](auto nc) {
         constexpr int N = decltype(nc)::value;
-        if constexpr (p1_teams<R, N>() == 1) {
-            const int items = (a.g.nkxh - 1) * (a.ph.linear ? 2 : 6) + (a.ph.linear ? 1 : 3);
-            if (dargs) hipLaunchKernelGGL((k_p1_row<R, N, true>), dim3(items, count), dim3(TeamShape<N>::TT), 0, s, a, dargs);
-            else hipLaunchKernelGGL((k_p1_row<R, N, false>), dim3(items), dim3(TeamShape<N>::TT), 0, s, a, dargs);
+        if (variant == kP1Row) {
+            const dim3 grid((a.g.nkxh - 1) * (a.ph.linear ? 2 : 6) + (a.ph.linear ? 1 : 3), dargs ? count : 1);
+            if (dargs) hipLaunchKernelGGL((k_p1_row<R, N, true>), grid, dim3(TeamShape<N>::TT), 0, s, a, dargs);
+            else hipLaunchKernelGGL((k_p1_row<R, N, false>), grid, dim3(TeamShape<N>::TT), 0, s, a, dargs);
         } else {
             const P1Grid pg(a.g.nkxh, a.ph.linear);
-            if (dargs) hipLaunchKernelGGL((k_p1<R, N, true>), dim3(pg.grid, count), dim3(kTile * TeamShape<N>::TT), 0, s, a, dargs);
-            else hipLaunchKernelGGL((k_p1<R, N, false>), dim3(pg.grid), dim3(kTile * TeamShape<N>::TT), 0, s, a, dargs);
+            const dim3 grid(pg.grid, dargs ? count : 1), block(kTile * TeamShape<N>::TT);
+            if (dargs) hipLaunchKernelGGL((k_p1<R, N, true>), grid, block, 0, s, a, dargs);
+            else hipLaunchKernelGGL((k_p1<R, N, false>), grid, block, 0, s, a, dargs);
         }
     });
     return ok ? hipGetLastError() : unsupported();
 }
 
 template <>
-hipError_t Launch<R>::p2(const StepArgs<R>& a, const StepArgs<R>* dargs, int count, hipStream_t s) {
+hipError_t Launch<R>::p2(const StepArgs<R>& a, const StepArgs<R>* dargs, int count, int variant, hipStream_t s) {
     bool ok = dispatch(a.g.nx, [&](auto nc) {
         constexpr int N = decltype(nc)::value;
         constexpr int KT = p2_teams<R, N>();
-        const dim3 block(KT * Team<N, p2_points<R, N>()>::TT);
-        if (dargs) hipLaunchKernelGGL((k_p2<R, N, true>), dim3(a.g.ny / KT, count), block, 0, s, a, dargs);
-        else hipLaunchKernelGGL((k_p2<R, N, false>), dim3(a.g.ny / KT), block, 0, s, a, dargs);
+        const dim3 grid(a.g.ny / KT, dargs ? count : 1), block(KT * Team<N, p2_points<R, N>()>::TT);
+        if (variant == kP2Three) {
+            if (dargs) hipLaunchKernelGGL((k_p2<R, N, true, true>), grid, block, 0, s, a, dargs);
+            else hipLaunchKernelGGL((k_p2<R, N, false, true>), grid, block, 0, s, a, dargs);
+        } else {
+            if (dargs) hipLaunchKernelGGL((k_p2<R, N, true, false>), grid, block, 0, s, a, dargs);
+            else hipLaunchKernelGGL((k_p2<R, N, false, false>), grid, block, 0, s, a, dargs);
+        }
     });
     return ok ? hipGetLastError() : unsupported();
 }
 
 template <>
-hipError_t Launch<R>::p3(const StepArgs<R>& a, const StepArgs<R>* dargs, int count, hipStream_t s) {
+hipError_t Launch<R>::p3(const StepArgs<R>& a, const StepArgs<R>* dargs, int count, int variant, hipStream_t s) {
     bool ok = dispatch(a.g.ny, [&](auto nc) {
         constexpr int N = decltype(nc)::value;
-        constexpr int KT = p3_teams<R, N>();
-        const int groups = (a.g.nkxh + KT - 1) / KT;
-        if (dargs) hipLaunchKernelGGL((k_p3<R, N, true>), dim3(2 * groups, count), dim3(KT * TeamShape<N>::TT), 0, s, a, dargs);
-        else hipLaunchKernelGGL((k_p3<R, N, false>), dim3(2 * groups), dim3(KT * TeamShape<N>::TT), 0, s, a, dargs);
+        auto go = [&](auto ktc) {
+            constexpr int KT = decltype(ktc)::value;
+            const dim3 grid(2 * ((a.g.nkxh + KT - 1) / KT), dargs ? count : 1), block(KT * TeamShape<N>::TT);
+            if (dargs) hipLaunchKernelGGL((k_p3<R, N, true, KT>), grid, block, 0, s, a, dargs);
+            else hipLaunchKernelGGL((k_p3<R, N, false, KT>), grid, block, 0, s, a, dargs);
+        };
+        if (variant == kP3Pair) go(std::integral_constant<int, 1>{});
+        else go(std::integral_constant<int, kTile>{});
     });
     return ok ? hipGetLastError() : unsupported();
 }

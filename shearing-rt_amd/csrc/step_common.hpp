@@ -34,19 +34,21 @@ constexpr int waves_per_eu() {
 
 // Points per thread of each step kernel.
 template <typename T, int N> constexpr int p1_points() { return TeamShape<N>::P; }
-// Rows (teams) per P1 workgroup: 1 (one transform per workgroup) or kTile.
-template <typename T, int N> constexpr int p1_teams() { return 1; }
 template <typename T, int N> constexpr int p2_points() { return TeamShape<N>::P; }
-// P2 with three x-inverse transforms (three real fields live across the last)
-// instead of four (two live): one 2048-point transform fewer per column, at
-// the price of ~9 spilled registers at 2048 (34.4 against 36.8 us per launch at
-// 2048^2 FP32).
-template <typename T, int N> constexpr bool p2_three_pass() { return true; }
 // Columns (teams) per P2 workgroup: 1, 2 or kTile.
 template <typename T, int N> constexpr int p2_teams() { return 1; }
 template <typename T, int N> constexpr int p3_points() { return TeamShape<N>::P; }
-// Row pairs (teams) per P3 workgroup: 1 or kTile.
-template <typename T, int N> constexpr int p3_teams() { return 1; }
+
+// Kernel variants chosen per context at run time (shrt_autotune): one code per
+// pass.  Defaults: the fastest at 2048^2 FP32 (P1 31.3 us row against 32.4 us
+// tile; P2 34.4 us three transforms against 36.8 us four; P3 29.7 us one row
+// pair against 32.0 us four).
+enum : int { kP1Row = 0, kP1Tile = 1 };        // one (row pair, output) / a tile of kTile row pairs per workgroup
+enum : int { kP2Three = 0, kP2Four = 1 };      // x-inverse transforms per column
+enum : int { kP3Pair = 0, kP3Tile = 1 };       // one row pair / kTile row pairs per workgroup
+struct Variants {
+    int p1 = kP1Row, p2 = kP2Three, p3 = kP3Pair;
+};
 
 // Exchange synchronisation of a team: a wave-level fence for one-wave teams,
 // otherwise a workgroup barrier.

@@ -296,7 +296,7 @@ k_p1_row(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
 // Workgroups of P2: KT = p2_teams columns each (a quarter, half or all of a
 // G tile of kTile columns); the workgroups of one tile run on one XCD so the
 // tile's lines are fetched once into its L2.
-template <typename T, int NX, bool BATCH>
+template <typename T, int NX, bool BATCH, bool THREE>
 __global__ void __launch_bounds__((p2_teams<T, NX>() * Team<NX, p2_points<T, NX>()>::TT),
                                   (waves_per_eu<T, NX, Team<NX, p2_points<T, NX>()>::TT, p2_teams<T, NX>()>()))
 k_p2(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
@@ -340,7 +340,6 @@ k_p2(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
     // rather than held in registers across transforms.
     // THREE: the pairs (vx, vy), (dw/dx, dw/dy), (dr/dx, dr/dy), one transform
     // fewer, with vx, vy and B_omega live across the last one.
-    constexpr bool THREE = p2_three_pass<T, NX>();
     T bw[P], br[P];
     T vxs[THREE ? P : 1];
     const int npass = a.ph.linear ? 1 : (THREE ? 3 : 4);
@@ -603,16 +602,14 @@ __device__ void p3_bookkeeping(const StepArgs<T>& a, double dt, int h, int which
     }
 }
 
-// Workgroups of P3: (group of KT row pairs, field), KT = p3_teams.  One row
+// Workgroups of P3: (group of KT row pairs, field), KT = 1 or kTile.  One row
 // pair per workgroup (KT = 1) gives eight small workgroups per CU instead of
 // two large ones, so their load, transform and update phases overlap.
-template <typename T, int NY, bool BATCH>
-__global__ void __launch_bounds__((p3_teams<T, NY>() * TeamShape<NY>::TT),
-                                  (waves_per_eu<T, NY, TeamShape<NY>::TT, p3_teams<T, NY>()>()))
+template <typename T, int NY, bool BATCH, int KT>
+__global__ void __launch_bounds__((KT * TeamShape<NY>::TT), (waves_per_eu<T, NY, TeamShape<NY>::TT, KT>()))
 k_p3(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
     const StepArgs<T>& a = step_args_of<BATCH>(a1, args);
     using S = TeamShape<NY>;
-    constexpr int KT = p3_teams<T, NY>();
     constexpr int P = S::P, TT = S::TT, L = lds_len<NY>();
     constexpr int NT = KT * TT;
     constexpr int LP = stage_stride<NY>();

@@ -59,6 +59,10 @@ class Diag(ctypes.Structure):
     _fields_ = [("energy", ctypes.c_double), ("enstrophy", ctypes.c_double), ("rho_var", ctypes.c_double)]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("p1", ctypes.c_int), ("p2", ctypes.c_int), ("p3", ctypes.c_int), ("ms", ctypes.c_double * 6)]
+
+
 class Halving(ctypes.Structure):
     _fields_ = [("istep", ctypes.c_int64), ("time", ctypes.c_double), ("value", ctypes.c_double),
                 ("delt", ctypes.c_double), ("which", ctypes.c_int), ("count", ctypes.c_int),
@@ -83,6 +87,8 @@ _SIGS = {
     "shrt_xtok": ([_VP, _VP, _VP], ctypes.c_int),
     "shrt_ktox": ([_VP, _VP, _VP], ctypes.c_int),
     "shrt_ktox_shear": ([_VP, _VP, _VP], ctypes.c_int),
+    "shrt_autotune": ([_VP, ctypes.c_int, ctypes.POINTER(Tuning)], ctypes.c_int),
+    "shrt_set_variants": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "shrt_ensemble_create": ([ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int,
                               ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
     "shrt_ensemble_destroy": ([_VP], ctypes.c_int),
@@ -281,6 +287,14 @@ class Shrt:
         out = np.zeros(3)
         _check(lib().shrt_kernel_times(self.h, int(n), _ptr(out)))
         return out
+
+    def autotune(self, reps=20):
+        t = Tuning()
+        _check(lib().shrt_autotune(self.h, int(reps), ctypes.byref(t)))
+        return dict(p1=t.p1, p2=t.p2, p3=t.p3, ms=list(t.ms))
+
+    def set_variants(self, p1, p2, p3):
+        _check(lib().shrt_set_variants(self.h, int(p1), int(p2), int(p3)))
 
     def halvings(self, cap=256):
         arr = (Halving * cap)()

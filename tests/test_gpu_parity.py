@@ -307,3 +307,33 @@ def test_lab_frame_output(gpu, prec):
     assert g.state()["remap_count"] == 1
     with pytest.raises(shrt.ShrtError):
         g.physical(shrt.P_VX, lab_frame=1)
+
+
+@pytest.mark.parametrize("prec", [8, 4])
+def test_kernel_variants(gpu, prec):
+    """Every combination of the pass variants matches the oracle and the others
+    (to rounding: the variants are compiled in different kernels, so the
+    compiler may contract products differently); autotune picks valid ones."""
+    n, steps = 128, 40
+    print(f"variants prec={prec}", flush=True)
+    ic = dict(seed=31, kmin=2.0, kmax=10.0, amp=1e-3, expo=-1.0)
+    o = oracle.Oracle(nx=n, ny=n, **dict(BASE, sigma=1.0))
+    o.set_spectral("rho", oracle.seeded_spectrum(n, n, 2 * np.pi, 2 * np.pi, **ic))
+    o.step(steps)
+    ref = o.buffer("arho")
+    outs = {}
+    for v in [(a, b, c) for a in (0, 1) for b in (0, 1) for c in (0, 1)]:
+        g = shrt.Shrt(nx=n, ny=n, precision=prec, graph_steps=4, **dict(BASE, sigma=1.0))
+        g.set_variants(*v)
+        g.initialize_seeded(**ic)
+        g.time_advance(steps)
+        outs[v] = g.spectral(shrt.RHO)
+        assert rel_l2(outs[v], ref) < TOLN[prec], v
+        g.close()
+    for v, r in outs.items():
+        assert rel_l2(r, outs[(0, 0, 0)]) < TOLN[prec], v
+    g = shrt.Shrt(nx=n, ny=n, precision=prec, **dict(BASE, sigma=1.0))
+    t = g.autotune(reps=5)
+    assert t["p1"] in (0, 1) and t["p2"] in (0, 1) and t["p3"] in (0, 1)
+    assert all(m > 0 for m in t["ms"])
+    g.close()
