@@ -168,6 +168,9 @@ def main():
                                                        "CAL_WRITE_CSV", "CAL_BYTES"),
                     help="only turn rocprofv3 FETCH_SIZE/WRITE_SIZE counter files into HBM bytes per "
                          "dispatch (JSON on stdout) and exit")
+    ap.add_argument("--autotune-report", default="", metavar="SIZES",
+                    help="only run shrt_autotune for each square size (e.g. 256,512,1024,2048,4096) and "
+                         "print the per-variant kernel times as JSON lines, then exit")
     ap.add_argument("--ensemble", default="", metavar="SPEC",
                     help="only time an ensemble, e.g. 128x24,256x24,512x16 (grid x count; FP32, mixed "
                          "shear rates and seeds, one captured graph per --graph-steps steps) and exit")
@@ -203,6 +206,18 @@ def main():
 
     if args.ensemble:
         run_ensemble(shrt, args)
+        return
+    if args.autotune_report:
+        prec = shrt.FP32 if args.precision == "fp32" else shrt.FP64
+        for n in (int(x) for x in args.autotune_report.split(",")):
+            g = shrt.Shrt(nx=n, ny=n, precision=prec, graph_steps=0, Lx=2 * np.pi, Ly=2 * np.pi, delt=1e-4,
+                          cfl_num=0.35, nu=1e-4, kappa=1e-4, sigma=1.0, g=1.0, rho0=1.0, rho0_prime=1.0,
+                          rho_eps1=1e-2, noise=0, linear=0)
+            t = g.autotune(reps=20)
+            names = ["p1_row", "p1_tile", "p2_three", "p2_four", "p3_pair", "p3_tile"]
+            print(json.dumps({"n": n, "precision": args.precision, "choice": [t["p1"], t["p2"], t["p3"]],
+                              "us_per_launch": {k: round(v * 1e3, 2) for k, v in zip(names, t["ms"])}}), flush=True)
+            g.close()
         return
 
     n = args.n
