@@ -160,6 +160,8 @@ def main():
     ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
     ap.add_argument("--graph-steps", type=int, default=20)
     ap.add_argument("--kernel-steps", type=int, default=50)
+    ap.add_argument("--fusion", type=int, choices=[0, 1], default=0,
+                    help="captured batches fuse P3 of step n with P1 of step n+1 (1) or not (0, default)")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU-baseline work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="BYTES",
@@ -216,7 +218,10 @@ def main():
             t = g.autotune(reps=20)
             names = ["p1_row", "p1_tile", "p2_three", "p2_four", "p3_pair", "p3_tile"]
             print(json.dumps({"n": n, "precision": args.precision, "choice": [t["p1"], t["p2"], t["p3"]],
-                              "us_per_launch": {k: round(v * 1e3, 2) for k, v in zip(names, t["ms"])}}), flush=True)
+                              "us_per_launch": {k: round(v * 1e3, 2) for k, v in zip(names, t["ms"])},
+                              "fuse": t["fuse"], "us_per_step": {"unfused": round(t["step_ms"][0] * 1e3, 2),
+                                                                 "fused": round(t["step_ms"][1] * 1e3, 2)}}),
+                  flush=True)
             g.close()
         return
 
@@ -228,6 +233,7 @@ def main():
 
     log(f"bench: {n}^2 {args.precision}, graph_steps={args.graph_steps}, warmup={args.warmup}, steps={args.steps}")
     g = shrt.Shrt(nx=n, ny=n, precision=prec, graph_steps=args.graph_steps, **params)
+    g.set_fusion(args.fusion)
     g.initialize_seeded(**ic)
     if args.warmup > 0:
         g.time_advance(args.warmup)
@@ -270,7 +276,7 @@ def main():
         "data": "synthetic: omega=0, seeded random-phase rho on 2<=|k|<=32, amplitude 1e-4|k|^-1.5",
         "config": {"workload": f"pseudospectral step, {n}^2 {args.precision.upper()} single instance "
                                "(sigma=1, nu=kappa=1e-4, g=1, delt=1e-4)",
-                   "nx": n, "ny": n, "graph_steps": args.graph_steps},
+                   "nx": n, "ny": n, "graph_steps": args.graph_steps, "fusion": args.fusion},
         "roofline": {"bound": "hbm", "kernel": dname, "achieved": achieved, "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
                      "bytes_per_launch": kb[dname], "ms_per_launch": float(kt[dom])},

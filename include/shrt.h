@@ -156,8 +156,11 @@ shrt_status shrt_ktox_shear(shrt_ctx* ctx, const void* spec_in, void* phys_out);
  * jump[nkx] (ikx_indexed fold for jump > 0). */
 shrt_status shrt_shearing_field(shrt_ctx* ctx, const int* jump, const void* in, void* out);
 
-/* Time and shift bookkeeping: time / istep / delt, and ky_shift[nkx] (FP64) plus the
- * jump[nkx] of the last remap (shear.cu:21-23). */
+/* Time and shift bookkeeping: time / istep / delt, and ky_shift[nkx] (FP64) plus
+ * jump[nkx] (shear.cu:21-23) as shearing_ky leaves it after the last step
+ * (shear.cu:275-294): round(ky_shift/kx[1]) before a remap zeroed the shift,
+ * i.e. the remap's table right after a remap step and the current shift's
+ * otherwise; all zero before the first step. */
 shrt_status shrt_get_time(shrt_ctx* ctx, shrt_time* out);
 shrt_status shrt_set_time(shrt_ctx* ctx, int64_t istep, double time, double delt);
 shrt_status shrt_get_shift(shrt_ctx* ctx, double* ky_shift, int* jump);
@@ -194,13 +197,27 @@ shrt_status shrt_copy_bench(long long bytes, int reps, double* ms);
  * parameters (reps launches each) and keeps the faster.  The variants agree
  * to rounding (the P2 variants pair the fields differently in the complex
  * transforms; all are compiled separately).  ms[] = average ms per launch of
- * p1 row, p1 tile, p2 three, p2 four, p3 pair, p3 tile. */
+ * p1 row, p1 tile, p2 three, p2 four, p3 pair, p3 tile.  It then times
+ * captured batches of graph_steps steps (20 if the context has none) with and
+ * without the P3+P1 fusion (shrt_set_fusion) and keeps the faster. */
 typedef struct {
     int p1, p2, p3;
     double ms[6];
+    int fuse;            /* 1: captured batches fuse P3 with the next P1 (shrt_set_fusion) */
+    double step_ms[2];   /* ms per step of captured batches, unfused and fused */
 } shrt_tuning;
 shrt_status shrt_autotune(shrt_ctx* ctx, int reps, shrt_tuning* out);
 shrt_status shrt_set_variants(shrt_ctx* ctx, int p1, int p2, int p3);
+
+/* Step fusion (new; no reference counterpart).  With on = 1 a captured batch
+ * of graph_steps steps runs P3 of step n and P1 of step n+1 as one kernel: the
+ * updated rows stay in LDS, the next shift and remap are formed in the kernel,
+ * and the next step's y-inverse transforms are written without reading the new
+ * state back.  on = 0 (the default) runs the three passes per step.  Single
+ * steps (shrt_step with fewer than graph_steps left) never fuse.  Fusion saves
+ * P1's reads but gives a row pair's workgroup its transforms in series, so it
+ * pays on some shapes only; shrt_autotune times both and keeps the faster. */
+shrt_status shrt_set_fusion(shrt_ctx* ctx, int on);
 
 /* Ensembles (new; no reference counterpart: the reference runs one instance
  * per process).  Independent contexts -- any sizes, precisions, parameters and

@@ -22,6 +22,8 @@ struct Launch {
     static hipError_t p1(const StepArgs<T>& geom, const StepArgs<T>* dargs, int count, int variant, hipStream_t s);
     static hipError_t p2(const StepArgs<T>& geom, const StepArgs<T>* dargs, int count, int variant, hipStream_t s);
     static hipError_t p3(const StepArgs<T>& geom, const StepArgs<T>* dargs, int count, int variant, hipStream_t s);
+    // P3 of step n fused with P1 of step n+1 (one row pair per workgroup).
+    static hipError_t p31(const StepArgs<T>& geom, const StepArgs<T>* dargs, int count, hipStream_t s);
     static hipError_t xfwd(const AuxArgs<T>& a, hipStream_t s);
     static hipError_t yfwd(const AuxArgs<T>& a, hipStream_t s);
     static hipError_t yinv(const AuxArgs<T>& a, hipStream_t s);

@@ -110,7 +110,7 @@ bool prec_supported(int prec, int n) {
 // argument blocks
 // ---------------------------------------------------------------------------
 template <typename T>
-StepArgs<T> step_args(shrt_ctx* c, int in_par, int store_phi) {
+StepArgs<T> step_args(shrt_ctx* c, int in_par, int store_phi, int store_phi_next = 0) {
     StepArgs<T> a{};
     a.g = c->g;
     a.ph = c->ph;
@@ -135,6 +135,7 @@ StepArgs<T> step_args(shrt_ctx* c, int in_par, int store_phi) {
     a.hlog = c->hlog;
     a.hlog_cap = kHalvingCap;
     a.store_phi = store_phi;
+    a.store_phi_next = store_phi_next;
     a.cflp = c->cflp;
     a.ncflp = Launch<T>::p2_workgroups(c->g.nx, c->g.ny);
     return a;
@@ -169,13 +170,36 @@ hipError_t enqueue_step(shrt_ctx* c, int in_par, int store_phi) {
     return c->prec == 4 ? enqueue_step_t<float>(c, in_par, store_phi) : enqueue_step_t<double>(c, in_par, store_phi);
 }
 
+// Step k of a captured batch of K >= 2 steps.  Fused: step 0 runs P1, P2 and
+// P3+P1 of step 1; steps 1..K-2 run P2 and P3+P1 of the next step; step K-1
+// runs P2 and P3.  aphi is stored for the batch's last step (by the fused
+// kernel of step K-2), as the unfused batch stores it in that step's P1.
+template <typename T>
+hipError_t enqueue_batch_step_t(shrt_ctx* c, int in_par, int k, int K, hipStream_t s) {
+    const bool last = k == K - 1;
+    if (!c->var.fuse) {
+        const StepArgs<T> a = step_args<T>(c, in_par, last);
+        hipError_t e = Launch<T>::p1(a, nullptr, 1, c->var.p1, s);
+        if (e == hipSuccess) e = Launch<T>::p2(a, nullptr, 1, c->var.p2, s);
+        if (e == hipSuccess) e = Launch<T>::p3(a, nullptr, 1, c->var.p3, s);
+        return e;
+    }
+    const StepArgs<T> a = step_args<T>(c, in_par, 0, k + 1 == K - 1);
+    hipError_t e = k == 0 ? Launch<T>::p1(a, nullptr, 1, c->var.p1, s) : hipSuccess;
+    if (e == hipSuccess) e = Launch<T>::p2(a, nullptr, 1, c->var.p2, s);
+    if (e == hipSuccess) e = last ? Launch<T>::p3(a, nullptr, 1, c->var.p3, s) : Launch<T>::p31(a, nullptr, 1, s);
+    return e;
+}
+
 shrt_status build_graph(shrt_ctx* c, int par) {
     if (c->gexec[par]) return SHRT_OK;
     const int K = c->p.graph_steps;
     hipGraph_t graph = nullptr;
     HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     hipError_t err = hipSuccess;
-    for (int k = 0; k < K && err == hipSuccess; ++k) err = enqueue_step(c, par ^ (k & 1), k == K - 1);
+    for (int k = 0; k < K && err == hipSuccess; ++k)
+        err = c->prec == 4 ? enqueue_batch_step_t<float>(c, par ^ (k & 1), k, K, c->stream)
+                           : enqueue_batch_step_t<double>(c, par ^ (k & 1), k, K, c->stream);
     hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
     if (err != hipSuccess) return fail(SHRT_ERR_DEVICE, std::string("capture: ") + hipGetErrorString(err));
     if (e2 != hipSuccess) return fail(SHRT_ERR_DEVICE, std::string("end capture: ") + hipGetErrorString(e2));
@@ -673,7 +697,8 @@ shrt_status shrt_autotune(shrt_ctx* c, int reps, shrt_tuning* out) {
     // a scratch instance of the same shape and parameters, a few steps into a seeded run
     shrt_ctx* t = nullptr;
     shrt_params p = c->p;
-    p.graph_steps = 0;
+    const int K = c->p.graph_steps >= 2 ? c->p.graph_steps : 20;
+    p.graph_steps = K;
     shrt_status st = shrt_create(&p, &t);
     if (st != SHRT_OK) return st;
     shrt_seeded_ic ic{12345u, 2.0, std::max(2.0, c->g.ny / 16.0), 1e-4, -1.5};
@@ -685,11 +710,26 @@ shrt_status shrt_autotune(shrt_ctx* c, int reps, shrt_tuning* out) {
         for (int v = 0; v < nvar[pass] && st == SHRT_OK; ++v)
             st = c->prec == 4 ? time_pass<float>(t, pass, v, reps, &ms[pass][v])
                               : time_pass<double>(t, pass, v, reps, &ms[pass][v]);
+    Variants chosen;
+    chosen.p1 = ms[0][1] < ms[0][0] ? kP1Tile : kP1Row;
+    chosen.p2 = ms[1][1] < ms[1][0] ? kP2Four : kP2Three;
+    chosen.p3 = ms[2][1] < ms[2][0] ? kP3Tile : kP3Pair;
+    // captured batches with and without the P3+P1 fusion, alternated, best of two each
+    double step_ms[2] = {1e300, 1e300};
+    const int nb = std::max(2, reps / 4);
+    for (int round = 0; round < 2 && st == SHRT_OK; ++round)
+        for (int f = 0; f < 2 && st == SHRT_OK; ++f) {
+            t->var = chosen;
+            st = shrt_set_fusion(t, f);
+            double ms_total = 0.0;
+            if (st == SHRT_OK) st = shrt_step(t, K);   // capture and warm
+            if (st == SHRT_OK) st = shrt_time_steps(t, K * nb, &ms_total);
+            if (st == SHRT_OK) step_ms[f] = std::min(step_ms[f], ms_total / (K * nb));
+        }
     (void)shrt_destroy(t);
     if (st != SHRT_OK) return st;
-    c->var.p1 = ms[0][1] < ms[0][0] ? kP1Tile : kP1Row;
-    c->var.p2 = ms[1][1] < ms[1][0] ? kP2Four : kP2Three;
-    c->var.p3 = ms[2][1] < ms[2][0] ? kP3Tile : kP3Pair;
+    chosen.fuse = step_ms[1] < step_ms[0] ? 1 : 0;
+    c->var = chosen;
     for (int i = 0; i < 2; ++i)   // captured graphs hold the old launches
         if (c->gexec[i]) {
             (void)hipGraphExecDestroy(c->gexec[i]);
@@ -701,6 +741,9 @@ shrt_status shrt_autotune(shrt_ctx* c, int reps, shrt_tuning* out) {
         out->p3 = c->var.p3;
         for (int pass = 0; pass < 3; ++pass)
             for (int v = 0; v < 2; ++v) out->ms[2 * pass + v] = ms[pass][v];
+        out->fuse = c->var.fuse;
+        out->step_ms[0] = step_ms[0];
+        out->step_ms[1] = step_ms[1];
     }
     return SHRT_OK;
 }
@@ -710,6 +753,17 @@ shrt_status shrt_set_variants(shrt_ctx* c, int p1, int p2, int p3) {
     c->var.p1 = p1;
     c->var.p2 = p2;
     c->var.p3 = p3;
+    for (int i = 0; i < 2; ++i)
+        if (c->gexec[i]) {
+            (void)hipGraphExecDestroy(c->gexec[i]);
+            c->gexec[i] = nullptr;
+        }
+    return SHRT_OK;
+}
+
+shrt_status shrt_set_fusion(shrt_ctx* c, int on) {
+    if (!c || on < 0 || on > 1) return fail(SHRT_ERR_INVALID, "bad fusion flag");
+    c->var.fuse = on;
     for (int i = 0; i < 2; ++i)
         if (c->gexec[i]) {
             (void)hipGraphExecDestroy(c->gexec[i]);
@@ -794,15 +848,22 @@ shrt_status shrt_get_shift(shrt_ctx* c, double* ky_shift, int* jump) {
     StepCtl v;
     shrt_status st = read_ctl(c, &v);
     if (st != SHRT_OK) return st;
-    if (ky_shift)
-        HIP_TRY(hipMemcpyAsync(ky_shift, c->s + (size_t)c->cur * c->g.nkx, sizeof(double) * c->g.nkx,
+    std::vector<double> s(c->g.nkx);
+    HIP_TRY(hipMemcpyAsync(s.data(), c->s + (size_t)c->cur * c->g.nkx, sizeof(double) * c->g.nkx,
+                           hipMemcpyDeviceToHost, c->stream));
+    // shearing_ky (shear.cu:280-281) sets jump = round(shift/kx[1]) every step,
+    // before a remap zeroes the shift: after a remap step that is the remap's
+    // table, otherwise it follows from the current shift; row 0 keeps 0.
+    const bool remapped = v.remap_count > 0 && v.last_remap_step == v.istep - 1;
+    if (jump && remapped)
+        HIP_TRY(hipMemcpyAsync(jump, c->jump + (size_t)(v.epoch & 1) * c->g.nkx, sizeof(int) * c->g.nkx,
                                hipMemcpyDeviceToHost, c->stream));
-    if (jump) {
-        if (v.epoch == 0) memset(jump, 0, sizeof(int) * c->g.nkx);
-        else HIP_TRY(hipMemcpyAsync(jump, c->jump + (size_t)(v.epoch & 1) * c->g.nkx, sizeof(int) * c->g.nkx,
-                                    hipMemcpyDeviceToHost, c->stream));
-    }
     HIP_TRY(hipStreamSynchronize(c->stream));
+    if (ky_shift) memcpy(ky_shift, s.data(), sizeof(double) * c->g.nkx);
+    if (jump && !remapped) {
+        jump[0] = 0;
+        for (int i = 1; i < c->g.nkx; ++i) jump[i] = v.istep == 0 ? 0 : (int)std::round(s[i] / c->kx_h[1]);
+    }
     return SHRT_OK;
 }
 
@@ -1032,16 +1093,19 @@ struct shrt_ensemble {
 
 namespace {
 
-// The argument blocks of one group for every (step parity, store_phi): member
-// i at block [(2*k1 + sp) * n + i] steps from parity par_i ^ k1.
+// The argument blocks of one group for every (step parity, aphi flag): member
+// i at block [(2*k1 + sp) * n + i] steps from parity par_i ^ k1; sp is
+// store_phi, or store_phi_next when the group runs fused steps.
 template <typename T>
 hipError_t group_args(shrt_ensemble* e, const std::vector<int>& grp, const std::vector<int>& par, void** out) {
     const size_t n = grp.size();
+    const bool fuse = e->m[grp[0]]->var.fuse;
     std::vector<StepArgs<T>> h(4 * n);
     for (int k1 = 0; k1 < 2; ++k1)
         for (int sp = 0; sp < 2; ++sp)
             for (size_t i = 0; i < n; ++i)
-                h[(2 * k1 + sp) * n + i] = step_args<T>(e->m[grp[i]], par[grp[i]] ^ k1, sp);
+                h[(2 * k1 + sp) * n + i] = fuse ? step_args<T>(e->m[grp[i]], par[grp[i]] ^ k1, 0, sp)
+                                                : step_args<T>(e->m[grp[i]], par[grp[i]] ^ k1, sp, 0);
     hipError_t err = hipMalloc(out, sizeof(StepArgs<T>) * h.size());
     if (err != hipSuccess) return err;
     return hipMemcpy(*out, h.data(), sizeof(StepArgs<T>) * h.size(), hipMemcpyHostToDevice);
@@ -1050,14 +1114,17 @@ hipError_t group_args(shrt_ensemble* e, const std::vector<int>& grp, const std::
 template <typename T>
 hipError_t group_step(shrt_ensemble* e, const std::vector<int>& grp, const void* args, int k, hipStream_t s) {
     const size_t n = grp.size();
-    const int sp = k == e->K - 1;
+    const Variants& v = e->m[grp[0]]->var;   // the group runs its first member's variants
+    const bool last = k == e->K - 1;
+    const int sp = v.fuse ? (k + 1 == e->K - 1) : last;   // the batch layout of enqueue_batch_step_t
     const StepArgs<T>* d = static_cast<const StepArgs<T>*>(args) + (2 * (k & 1) + sp) * n;
     if (n == 1) d = nullptr;   // a group of one takes its arguments by value
-    const StepArgs<T> geom = step_args<T>(e->m[grp[0]], e->m[grp[0]]->cur ^ (k & 1), sp);
-    const Variants& v = e->m[grp[0]]->var;   // the group runs its first member's variants
-    hipError_t err = Launch<T>::p1(geom, d, (int)n, v.p1, s);
+    const StepArgs<T> geom = v.fuse ? step_args<T>(e->m[grp[0]], e->m[grp[0]]->cur ^ (k & 1), 0, sp)
+                                    : step_args<T>(e->m[grp[0]], e->m[grp[0]]->cur ^ (k & 1), sp, 0);
+    hipError_t err = (!v.fuse || k == 0) ? Launch<T>::p1(geom, d, (int)n, v.p1, s) : hipSuccess;
     if (err == hipSuccess) err = Launch<T>::p2(geom, d, (int)n, v.p2, s);
-    if (err == hipSuccess) err = Launch<T>::p3(geom, d, (int)n, v.p3, s);
+    if (err == hipSuccess)
+        err = (v.fuse && !last) ? Launch<T>::p31(geom, d, (int)n, s) : Launch<T>::p3(geom, d, (int)n, v.p3, s);
     return err;
 }
 

@@ -109,11 +109,22 @@ hipError_t Launch<R>::p3(const StepArgs<R>& a, const StepArgs<R>* dargs, int cou
         auto go = [&](auto ktc) {
             constexpr int KT = decltype(ktc)::value;
             const dim3 grid(2 * ((a.g.nkxh + KT - 1) / KT), dargs ? count : 1), block(KT * TeamShape<N>::TT);
-            if (dargs) hipLaunchKernelGGL((k_p3<R, N, true, KT>), grid, block, 0, s, a, dargs);
-            else hipLaunchKernelGGL((k_p3<R, N, false, KT>), grid, block, 0, s, a, dargs);
+            if (dargs) hipLaunchKernelGGL((k_p3<R, N, true, KT, false>), grid, block, 0, s, a, dargs);
+            else hipLaunchKernelGGL((k_p3<R, N, false, KT, false>), grid, block, 0, s, a, dargs);
         };
         if (variant == kP3Pair) go(std::integral_constant<int, 1>{});
         else go(std::integral_constant<int, kTile>{});
+    });
+    return ok ? hipGetLastError() : unsupported();
+}
+
+template <>
+hipError_t Launch<R>::p31(const StepArgs<R>& a, const StepArgs<R>* dargs, int count, hipStream_t s) {
+    bool ok = dispatch(a.g.ny, [&](auto nc) {
+        constexpr int N = decltype(nc)::value;
+        const dim3 grid(2 * a.g.nkxh, dargs ? count : 1), block(TeamShape<N>::TT);
+        if (dargs) hipLaunchKernelGGL((k_p3<R, N, true, 1, true>), grid, block, 0, s, a, dargs);
+        else hipLaunchKernelGGL((k_p3<R, N, false, 1, true>), grid, block, 0, s, a, dargs);
     });
     return ok ? hipGetLastError() : unsupported();
 }

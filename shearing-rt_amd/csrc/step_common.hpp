@@ -48,6 +48,7 @@ enum : int { kP2Three = 0, kP2Four = 1 };      // x-inverse transforms per colum
 enum : int { kP3Pair = 0, kP3Tile = 1 };       // one row pair / kTile row pairs per workgroup
 struct Variants {
     int p1 = kP1Row, p2 = kP2Three, p3 = kP3Pair;
+    int fuse = 0;   // captured batches run P3 of step n and P1 of step n+1 as one kernel
 };
 
 // Exchange synchronisation of a team: a wave-level fence for one-wave teams,
@@ -131,7 +132,8 @@ struct StepArgs {
     int* jump;
     HalvingRec* hlog;
     int hlog_cap;
-    int store_phi;
+    int store_phi;        // this step's P1 stores aphi
+    int store_phi_next;   // the fused next-step P1 (k_p3 with FUSE) stores aphi
     double* cflp;   // [ncflp][2]: per-workgroup max|vx + sigma*y|, max|vy| of P2
     int ncflp;      // P2 workgroups
 };

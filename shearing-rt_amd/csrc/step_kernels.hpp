@@ -571,7 +571,8 @@ __device__ void p3_bookkeeping(const StepArgs<T>& a, double dt, int h, int which
             o->slot_epoch_a[s] = s == s0 ? c->epoch : c->slot_epoch_a[s];
             o->slot_epoch_d[s] = s == s0 ? c->epoch : c->slot_epoch_d[s];
         }
-        o->phi_epoch = a.store_phi ? c->epoch : c->phi_epoch;
+        // aphi's frame: this step's P1 (store_phi) or, fused, the next step's (store_phi_next)
+        o->phi_epoch = a.store_phi_next ? newE : (a.store_phi ? c->epoch : c->phi_epoch);
         o->remap_count = c->remap_count + trig;
         o->last_remap_step = trig ? n : c->last_remap_step;
         o->halvings = c->halvings + h;
@@ -605,7 +606,16 @@ __device__ void p3_bookkeeping(const StepArgs<T>& a, double dt, int h, int which
 // Workgroups of P3: (group of KT row pairs, field), KT = 1 or kTile.  One row
 // pair per workgroup (KT = 1) gives eight small workgroups per CU instead of
 // two large ones, so their load, transform and update phases overlap.
-template <typename T, int NY, bool BATCH, int KT>
+//
+// FUSE (KT = 1): the workgroup goes on to the next step's P1 for its row pair
+// and field -- the "fused spectral kernel" of one step boundary.  The update
+// leaves the new rows in LDS (in place of the bracket values it consumed,
+// each position read and then written by the same thread; the two ky = 0
+// values in their own slots), the next shift and remap trigger follow from
+// the same FP64 operations as the bookkeeping, and the outputs of this field
+// (omega: vx, vy, d/dx, d/dy; rho: d/dx, d/dy) are transformed and written to
+// G through the remap view, as k_p1_row would after reading the rows back.
+template <typename T, int NY, bool BATCH, int KT, bool FUSE>
 __global__ void __launch_bounds__((KT * TeamShape<NY>::TT), (waves_per_eu<T, NY, TeamShape<NY>::TT, KT>()))
 k_p3(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
     const StepArgs<T>& a = step_args_of<BATCH>(a1, args);
@@ -617,8 +627,10 @@ k_p3(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
     constexpr int J = (NKY + TT - 1) / TT;          // row points per thread
     constexpr int JB = sizeof(T) == 4 ? 3 : 2;      // points whose loads are in flight together
     static_assert(KT == 1 || KT == kTile, "P3 workgroups hold one row pair or one H tile");
+    static_assert(!FUSE || KT == 1, "the fused next-step P1 works on one row pair");
     __shared__ cplx<T> sm[KT * L];
     __shared__ double red[32];
+    __shared__ cplx<T> slot[2];   // FUSE: the new ky = 0 values of the row and of its mirror
     const Geo& g = a.g;
     const Phys& ph = a.ph;
     const int team = threadIdx.x / TT, tid = threadIdx.x - team * TT;
@@ -719,8 +731,13 @@ k_p3(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
                 d0 = {nh.x - cc * phi.y, nh.y + cc * phi.x};
             }
             const long long at = (long long)r * g.nky + iky;
-            A0[at] = ab_bdf_update<T>(ist, plain, dt, diff, k2, B.a1[u], B.a2[u], d0, B.d1[u], B.d2[u]);
+            const cplx<T> an = ab_bdf_update<T>(ist, plain, dt, diff, k2, B.a1[u], B.a2[u], d0, B.d1[u], B.d2[u]);
+            A0[at] = an;
             D0[at] = d0;
+            if constexpr (FUSE) {
+                if (iky == 0) slot[side] = an;
+                else lds[side ? NY - iky : iky] = an;   // the position this thread just read
+            }
         }
     };
 
@@ -785,6 +802,110 @@ k_p3(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
     };
     if (ja1 != nullptr || ja2 != nullptr || jd1 != nullptr || jd2 != nullptr) run(std::true_type{});
     else run(std::false_type{});
+
+    if constexpr (FUSE) {
+        if (!act || (ph.linear && f == 1)) return;
+        __syncthreads();   // the new rows are in LDS
+        // shift and remap of this step for the two rows (p3_bookkeeping's operations)
+        const double sdt1 = g.nkx > 1 ? __dmul_rn(__dmul_rn(ph.sigma, a.kx[1]), dt) : 0.0;
+        const double s1n = g.nkx > 1 ? __dsub_rn(a.s_in[1], sdt1) : 0.0;
+        const bool trig = g.nkx > 1 && fabs(__dsub_rn(s1n, -a.kx[1])) < 1e-10;
+        double sn[2];
+        int jv[2];
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            const int r = side ? rn : row;
+            double v = s_r[side];
+            if (r > 0) v = __dsub_rn(s_r[side], __dmul_rn(__dmul_rn(ph.sigma, kx_r[side]), dt));
+            jv[side] = (trig && r > 0) ? (int)round(v / a.kx[1]) : 0;
+            sn[side] = (trig && r > 0) ? 0.0 : v;
+        }
+        // the new rows in the frame of step n+1 (view_load's remap on the LDS rows)
+        auto rowval = [&](int side, int ky) -> cplx<T> {
+            if (row == 0) side = 0;   // row 0 is its own mirror
+            if (ky == 0) return slot[side];
+            return side ? lds[NY - ky] : lds[ky];
+        };
+        auto viewval = [&](int side, int ky) -> cplx<T> {
+            if (!trig) return rowval(side, ky);
+            const int j = jv[side];
+            if (j <= 0) return ky - j < NKY ? rowval(side, ky - j) : czero<T>();
+            if (ky - j > 0) return rowval(side, ky - j);
+            return j - ky < NKY ? cconj(rowval(1 - side, j - ky)) : czero<T>();
+        };
+        // the spectrum of the row pair before multipliers: base[m] for e = tid + m*TT
+        cplx<T> base[P];
+#pragma unroll
+        for (int m = 0; m < P; ++m) {
+            const int e = tid + m * TT;
+            const bool pos = e < NKY;
+            const bool keep = pos || e > NY - NKY;
+            base[m] = keep ? viewval(pos ? 0 : 1, pos ? e : NY - e) : czero<T>();
+        }
+        const cplx<T> base_n0 = viewval(1, 0);   // the mirror row's ky = 0 value
+        __syncthreads();   // LDS is the transforms' scratch from here on
+        const int gp = g_pos_of(row, g.nkxh);
+        // One transform per output, in a rolled loop whose lane index is opaque
+        // so that the per-mode wavenumbers are recomputed per output rather
+        // than hoisted and kept live (48 more registers) across the transforms.
+        // Output oi multiplies X = phi (oi < 2) or the field by i*c:
+        // c = kys (0, 3, 5), -kx (1), kx (2, 4)  -- p1_mult in selected form.
+        const int o_end = f == 0 ? (ph.linear ? 2 : 4) : 6;
+#pragma nounroll
+        for (int oi = f == 0 ? 0 : 4; oi < o_end; ++oi) {
+            int tl = tid;
+            asm volatile("" : "+v"(tl));
+            tl &= TT - 1;
+            const bool use_phi = oi < 2, use_kys = oi == 0 || oi == 3 || oi == 5, neg = oi == 1;
+            auto mult = [&](const ModeK<T>& k, bool origin, cplx<T> F) -> cplx<T> {
+                if (use_phi) F = origin ? czero<T>() : cdivr(F, k.k2);
+                const T cf = use_kys ? k.kys : (neg ? -k.kx : k.kx);
+                return {-cf * F.y, cf * F.x};
+            };
+            cplx<T> v[P];
+#pragma unroll
+            for (int m = 0; m < P; ++m) {
+                const int e = tl + m * TT;
+                const bool pos = e < NKY;
+                const bool keep = pos || e > NY - NKY;
+                const int iky = keep ? (pos ? e : NY - e) : 0;
+                const int side = pos ? 0 : 1;
+                const int r = side ? rn : row;
+                const ModeK<T> k = mode_k<T>(a.ky, sn[side], kx_r[side], iky);
+                const cplx<T> z = mult(k, r == 0 && iky == 0, base[m]);
+                v[m] = keep ? (pos ? z : cplx<T>{z.x, -z.y}) : czero<T>();
+            }
+            if (tl == 0) {   // Hermitian part of the ky = 0 column
+                const ModeK<T> k = mode_k<T>(a.ky, sn[1], kx_r[1], 0);
+                const cplx<T> vn = mult(k, rn == 0, base_n0);
+                v[0] = {(v[0].x + vn.x) * T(0.5), (v[0].y - vn.y) * T(0.5)};
+            }
+            if (oi == 0 && a.store_phi_next) {   // aphi = omega/kperp2_shear of step n+1 (shear.cu:367-374)
+#pragma unroll
+                for (int m = 0; m < P; ++m) {
+                    const int e = tl + m * TT;
+                    const bool pos = e < NKY;
+                    if (!(pos || e > NY - NKY) || (!pos && row == 0)) continue;
+                    const int iky = pos ? e : NY - e;
+                    const int side = pos ? 0 : 1;
+                    const int r = side ? rn : row;
+                    const ModeK<T> k = mode_k<T>(a.ky, sn[side], kx_r[side], iky);
+                    a.aphi[(long long)r * g.nky + iky] = (r == 0 && iky == 0) ? czero<T>() : cdivr(base[m], k.k2);
+                }
+                if (tl == 0 && row != 0) {
+                    const ModeK<T> k = mode_k<T>(a.ky, sn[1], kx_r[1], 0);
+                    a.aphi[(long long)rn * g.nky] = cdivr(base_n0, k.k2);
+                }
+            }
+            TeamFFT<T, NY, P, S::RMAX, +1>::run(v, lds, a.twy, tid, sync);
+            cplx<T>* Gf = a.G + (long long)oi * g.I;
+#pragma unroll
+            for (int m = 0; m < P; ++m) {
+                const int y = tl + m * TT;
+                Gf[((long long)(y >> 2) * g.nkxh + gp) * 4 + (y & 3)] = v[m];
+            }
+        }
+    }
 }
 
 }  // namespace shrt

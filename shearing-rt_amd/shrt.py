@@ -60,7 +60,8 @@ class Diag(ctypes.Structure):
 
 
 class Tuning(ctypes.Structure):
-    _fields_ = [("p1", ctypes.c_int), ("p2", ctypes.c_int), ("p3", ctypes.c_int), ("ms", ctypes.c_double * 6)]
+    _fields_ = [("p1", ctypes.c_int), ("p2", ctypes.c_int), ("p3", ctypes.c_int), ("ms", ctypes.c_double * 6),
+                ("fuse", ctypes.c_int), ("step_ms", ctypes.c_double * 2)]
 
 
 class Halving(ctypes.Structure):
@@ -89,6 +90,7 @@ _SIGS = {
     "shrt_ktox_shear": ([_VP, _VP, _VP], ctypes.c_int),
     "shrt_autotune": ([_VP, ctypes.c_int, ctypes.POINTER(Tuning)], ctypes.c_int),
     "shrt_set_variants": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "shrt_set_fusion": ([_VP, ctypes.c_int], ctypes.c_int),
     "shrt_ensemble_create": ([ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int,
                               ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
     "shrt_ensemble_destroy": ([_VP], ctypes.c_int),
@@ -291,10 +293,14 @@ class Shrt:
     def autotune(self, reps=20):
         t = Tuning()
         _check(lib().shrt_autotune(self.h, int(reps), ctypes.byref(t)))
-        return dict(p1=t.p1, p2=t.p2, p3=t.p3, ms=list(t.ms))
+        return dict(p1=t.p1, p2=t.p2, p3=t.p3, ms=list(t.ms), fuse=t.fuse, step_ms=list(t.step_ms))
 
     def set_variants(self, p1, p2, p3):
         _check(lib().shrt_set_variants(self.h, int(p1), int(p2), int(p3)))
+
+    def set_fusion(self, on):
+        """Run captured batches with P3 of step n and P1 of step n+1 fused (default off)."""
+        _check(lib().shrt_set_fusion(self.h, 1 if on else 0))
 
     def halvings(self, cap=256):
         arr = (Halving * cap)()

@@ -19,18 +19,22 @@ SPECS = [  # nx, ny, precision, sigma, delt, seed
 ]
 
 
-def make(nx, ny, prec, sigma, delt, seed):
+def make(nx, ny, prec, sigma, delt, seed, graph_steps=0, fuse=True):
     p = dict(BASE)
     p.update(sigma=sigma, delt=delt, cfl_num=5.0 if delt > 5e-3 else 0.35)
-    g = shrt.Shrt(nx=nx, ny=ny, precision=prec, graph_steps=0, **p)
+    g = shrt.Shrt(nx=nx, ny=ny, precision=prec, graph_steps=graph_steps, **p)
+    g.set_fusion(fuse)
     g.initialize_seeded(seed=seed, kmin=2.0, kmax=8.0, amp=1e-3, expo=-1.0)
     return g
 
 
-def test_ensemble_equals_individual_runs(gpu):
-    print("ensemble of", len(SPECS), flush=True)
-    members = [make(*s) for s in SPECS]
-    solo = [make(*s) for s in SPECS]
+@pytest.mark.parametrize("fuse", [False, True])
+def test_ensemble_equals_individual_runs(gpu, fuse):
+    """Unfused: against single steps.  Fused: against each member's own
+    captured batches of the same length (the same fused kernel sequence)."""
+    print("ensemble of", len(SPECS), "fuse", fuse, flush=True)
+    members = [make(*s, fuse=fuse) for s in SPECS]
+    solo = [make(*s, graph_steps=4 if fuse else 0, fuse=fuse) for s in SPECS]
     ens = shrt.Ensemble(members, steps_per_graph=4)
     try:
         ens.step(102)           # 25 captured graphs, then 2 single steps per member

@@ -123,14 +123,20 @@ def test_multistep_sizes(gpu, prec, nx, ny, steps):
     o, g = pair(nx, prec, ny=ny, sigma=1.0)
     ic = dict(seed=4242, kmin=2.0, kmax=24.0, amp=1e-3, expo=-1.0)
     o.set_spectral("rho", oracle.seeded_spectrum(nx, ny, 2 * np.pi, 2 * np.pi, **ic))
-    g.initialize_seeded(**ic)
     o.step(steps)
-    g.time_advance(steps)
-    for f, ring in ((shrt.OMEGA, "aomg"), (shrt.RHO, "arho"), (shrt.PHI, "aphi")):
-        err = rel_l2(g.spectral(f), o.buffer(ring))
-        print(f"  field {f} rel err {err:.3e}")
-        assert err < TOLN[prec]
-    assert g.state()["istep"] == o.state()["istep"]
+    # single steps, and captured batches of two steps (P3 fused with the next P1)
+    for gs in (0, 2):
+        print(f"  graph_steps={gs}", flush=True)
+        g = shrt.Shrt(nx=nx, ny=ny, precision=prec, graph_steps=gs, **dict(BASE, sigma=1.0))
+        g.set_fusion(gs > 0)
+        g.initialize_seeded(**ic)
+        g.time_advance(steps)
+        for f, ring in ((shrt.OMEGA, "aomg"), (shrt.RHO, "arho"), (shrt.PHI, "aphi")):
+            err = rel_l2(g.spectral(f), o.buffer(ring))
+            print(f"  field {f} rel err {err:.3e}")
+            assert err < TOLN[prec]
+        assert g.state()["istep"] == o.state()["istep"]
+        g.close()
 
 
 @pytest.mark.parametrize("prec", [8, 4])
@@ -174,6 +180,49 @@ def test_remap_trigger(gpu, prec):
     g.time_advance(20)
     for f, ring in ((shrt.OMEGA, "aomg"), (shrt.RHO, "arho")):
         assert rel_l2(g.spectral(f), o.buffer(ring)) < TOLN[prec]
+
+
+@pytest.mark.parametrize("prec", [8, 4])
+@pytest.mark.parametrize("nx,ny,linear", [(32, 32, 0), (64, 128, 0), (128, 128, 0), (32, 32, 1)])
+def test_fused_batches_with_remaps(gpu, prec, nx, ny, linear):
+    """Captured batches run P3 of step n and P1 of step n+1 as one kernel.  With
+    sigma*delt*kx[1] = 0.2 a remap fires every fifth step (indices 4, 9, 14, 19):
+    at every position of a four-step batch, so the fused kernel forms the remap
+    view of the rows it has just updated (and the last step of a batch hands a
+    pending remap to the next batch's P1).  Oracle parity, index work
+    bit-exact, and the unfused batches and single steps agree to rounding."""
+    print(f"fused batches nx={nx} ny={ny} linear={linear} prec={prec}", flush=True)
+    p = dict(sigma=20.0, delt=0.01, cfl_num=1e3, nu=1e-2, kappa=1e-2, linear=linear)
+    o = oracle.Oracle(nx=nx, ny=ny, **dict(BASE, **p))
+    ic = dict(seed=17, kmin=1.0, kmax=8.0, amp=1e-3, expo=-1.0)
+    o.set_spectral("rho", oracle.seeded_spectrum(nx, ny, 2 * np.pi, 2 * np.pi, **ic))
+    steps = 22
+    o.step(steps)
+    so = o.state()
+    assert so["remap_count"] == 4 and so["last_remap_step"] == 19
+    outs = {}
+    for mode in ("fused", "unfused", "single"):
+        g = shrt.Shrt(nx=nx, ny=ny, precision=prec, graph_steps=0 if mode == "single" else 4, **dict(BASE, **p))
+        g.set_fusion(mode == "fused")
+        g.initialize_seeded(**ic)
+        g.time_advance(steps)
+        sg = g.state()
+        assert sg["remap_count"] == 4 and sg["last_remap_step"] == 19, mode
+        assert sg["istep"] == so["istep"] and sg["halvings"] == so["halvings"] == 0
+        sh_o, j_o = o.shift()
+        sh_g, j_g = g.shift()
+        assert np.array_equal(sh_o, sh_g) and np.array_equal(j_o, j_g), mode
+        outs[mode] = [g.spectral(f) for f in (shrt.OMEGA, shrt.RHO, shrt.PHI)]
+        for got, ring in zip(outs[mode], ("aomg", "arho", "aphi")):
+            err = rel_l2(got, o.buffer(ring))
+            print(f"  {mode} {ring} rel err {err:.3e}", flush=True)
+            assert err < TOLN[prec], (mode, ring, err)
+        g.close()
+    for mode in ("unfused", "single"):
+        for a, b in zip(outs["fused"], outs[mode]):
+            same = np.array_equal(a.view(np.uint8), b.view(np.uint8))
+            print(f"  fused vs {mode}: {'bit-identical' if same else f'rel {rel_l2(a, b):.2e}'}")
+            assert same or rel_l2(a, b) < TOL1[prec]
 
 
 @pytest.mark.parametrize("prec", [8, 4])
@@ -237,11 +286,13 @@ def test_linear_mode(gpu, prec):
 
 @pytest.mark.parametrize("prec", [8, 4])
 def test_graph_replay_matches_eager(gpu, prec):
+    """Unfused captured batches replay the single-step launches bit for bit."""
     n = 64
     ic = dict(seed=99, kmin=2.0, kmax=8.0, amp=1e-3, expo=-1.0)
     outs = []
     for gs in (0, 4):
         g = shrt.Shrt(nx=n, ny=n, precision=prec, graph_steps=gs, **BASE)
+        g.set_fusion(False)
         g.initialize_seeded(**ic)
         g.time_advance(9)
         g.time_advance(13)
@@ -332,8 +383,12 @@ def test_kernel_variants(gpu, prec):
         g.close()
     for v, r in outs.items():
         assert rel_l2(r, outs[(0, 0, 0)]) < TOLN[prec], v
-    g = shrt.Shrt(nx=n, ny=n, precision=prec, **dict(BASE, sigma=1.0))
+    g = shrt.Shrt(nx=n, ny=n, precision=prec, graph_steps=4, **dict(BASE, sigma=1.0))
     t = g.autotune(reps=5)
-    assert t["p1"] in (0, 1) and t["p2"] in (0, 1) and t["p3"] in (0, 1)
-    assert all(m > 0 for m in t["ms"])
+    assert t["p1"] in (0, 1) and t["p2"] in (0, 1) and t["p3"] in (0, 1) and t["fuse"] in (0, 1)
+    assert all(m > 0 for m in t["ms"]) and all(m > 0 for m in t["step_ms"])
+    assert t["fuse"] == (t["step_ms"][1] < t["step_ms"][0])
+    g.initialize_seeded(**ic)     # the tuned context steps (fused or not) and matches the oracle
+    g.time_advance(steps)
+    assert rel_l2(g.spectral(shrt.RHO), ref) < TOLN[prec]
     g.close()
