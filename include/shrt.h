@@ -235,6 +235,41 @@ shrt_status shrt_ensemble_synchronize(shrt_ensemble* ens);
 /* Timed run of nsteps (a multiple of steps_per_graph) between HIP events. */
 shrt_status shrt_ensemble_time_steps(shrt_ensemble* ens, int nsteps, double* ms_total);
 
+/* Learned closure (new; no reference counterpart).  A spectral-convolution
+ * network on an n x n grid (n a power of two, 8..512): inputs [batch][4][n*n]
+ * (channels omega, rho, phi, shear rate; physical layout f[ix*n + iy]),
+ * outputs [batch][2][n*n] (corrections of the omega and rho tendencies):
+ *   h0 = P x + p;  h_{l+1} = gelu(W_l h_l + w_l + K_l h_l), l < layers;
+ *   out = Q h_L + q,
+ * where K_l multiplies each retained mode (|kx| <= kmax, 0 <= ky <= kmax, half
+ * plane) of its input by a complex width x width matrix.  The truncated
+ * transforms, the mode mixing and the pointwise layers are GEMMs on the
+ * matrix cores with BF16 operands and FP32 accumulation; training (MSE loss,
+ * Adam) runs on the device on a data set uploaded once.  Parameters are one
+ * FP32 array: P[width][4], p[width]; per layer a[modes][width][width] (real
+ * parts, [out][in]), b[...] (imaginary parts), W[width][width], w[width]; then
+ * Q[2][width], q[2].  Modes are ordered ky = 0 with kx = 0..kmax, then each
+ * ky = 1..kmax with kx = -kmax..kmax. */
+typedef struct shrt_closure shrt_closure;
+shrt_status shrt_closure_create(int n, int kmax, int width, int layers, int max_batch, uint64_t seed,
+                                shrt_closure** out);
+shrt_status shrt_closure_destroy(shrt_closure* net);
+shrt_status shrt_closure_info(shrt_closure* net, long long* nparams, int* nmodes);
+shrt_status shrt_closure_get_params(shrt_closure* net, float* params);
+/* Sets the parameters and resets the optimiser state. */
+shrt_status shrt_closure_set_params(shrt_closure* net, const float* params);
+shrt_status shrt_closure_forward(shrt_closure* net, const float* in, int batch, float* out);
+/* Loss mean((out - target)^2) over batch*2*n*n values and its gradient with
+ * respect to every parameter (no update). */
+shrt_status shrt_closure_gradients(shrt_closure* net, const float* in, const float* target, int batch, double* loss,
+                                   float* grad);
+/* Upload a training set: in [nsamples][4][n*n], target [nsamples][2][n*n]. */
+shrt_status shrt_closure_set_data(shrt_closure* net, const float* in, const float* target, int nsamples);
+/* iters Adam steps (beta 0.9/0.999) on batches drawn with replacement from the
+ * training set by a seeded generator; losses[iters] receives each batch loss
+ * before its update. */
+shrt_status shrt_closure_train(shrt_closure* net, int iters, int batch, float lr, uint64_t seed, double* losses);
+
 /* Sizes derived by init_fft (fft.cu:91-96): out[0..7] = nkx, nky, nkxh, nkxh2,
  * nkxpad, ncy, nkx*nky, nkxh*ny. */
 shrt_status shrt_sizes(int nx, int ny, int* out8);

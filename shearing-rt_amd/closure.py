@@ -1,0 +1,257 @@
+"""Learned closure pipeline: training data from the solver's own fine runs, the
+spectral-convolution network trained on the device (shrt_closure_*), and the
+coarse solver corrected by it, with the plain coarse and a finer plain solver
+as the fallbacks it is judged against.
+
+Data.  A fine run (n_fine^2, FP32) is filtered to the coarse grid by keeping
+the coarse grid's retained modes (the same kx, ky, and the same shift since
+the shift depends on kx and time only).  For snapshots F_j taken every
+`interval` steps, the coarse solver restarted from F_j (Euler, AB2, AB3
+start-up) gives C_j after `interval` steps; the target is the coarse
+tendency defect (F_{j+1} - C_j) / (interval * delt) of omega and rho, and the
+inputs are the physical omega, rho, phi of F_j plus the shear rate.
+
+Use.  Every `interval` steps the corrected coarse run adds
+interval * delt * net(state) to omega and rho.  The correction is skipped
+(fallback to the plain coarse step) when the network output is not finite or
+larger than `guard` times the field it corrects.
+
+Judge.  The filtered fine run: rho relative L2 on the coarse modes and the
+relative L2 of the shell-summed enstrophy spectrum Z(k) = 1/2 sum |omega|^2
+(Hermitian-weighted) over the coarse shells.
+"""
+import time
+
+import numpy as np
+
+TWO_PI = 2 * np.pi
+
+
+class Config:
+    """One experiment.  Defaults: 128^2 fine, 32^2 coarse, RT-unstable
+    stratification sheared at sigma ~ U(0.5, 1.5), seeded rho perturbation on
+    1 <= |k| <= 4 grown into the nonlinear regime before snapshots start."""
+
+    def __init__(self, **kw):
+        self.n_fine = 128
+        self.n_coarse = 32
+        self.n_mid = 64              # the finer plain fallback
+        self.delt = 2e-3
+        self.interval = 10           # steps between snapshots / closure applications (even)
+        self.spinup = 1000           # fine steps before the first snapshot
+        self.snaps = 60              # snapshots per trajectory
+        self.nu = 1e-3
+        self.cfl = 5.0
+        self.amp, self.kmin, self.kmax_ic, self.expo = 0.05, 1.0, 4.0, -1.0
+        self.sigma_range = (0.5, 1.5)
+        self.kmax_net, self.width, self.layers = 10, 32, 4
+        self.guard = 0.5
+        for k, v in kw.items():
+            if not hasattr(self, k):
+                raise KeyError(k)
+            setattr(self, k, v)
+
+    def params(self, sigma):
+        return dict(Lx=TWO_PI, Ly=TWO_PI, delt=self.delt, cfl_num=self.cfl, nu=self.nu, kappa=self.nu,
+                    sigma=float(sigma), g=1.0, rho0=1.0, rho0_prime=1.0, rho_eps1=1e-2, noise=0, linear=0)
+
+    def sigma_of(self, seed):
+        lo, hi = self.sigma_range
+        return lo + (hi - lo) * np.random.default_rng(seed + 7919).random()
+
+
+def coarse_rows(s_fine, s_coarse):
+    """Rows of the fine spectrum holding the coarse grid's kx values (storage
+    order: kx = i for i < nkxh, kx = -(nkx - i) above)."""
+    nkx_c, nkxh_c, nkx_f = s_coarse["nkx"], s_coarse["nkxh"], s_fine["nkx"]
+    return np.array([i if i < nkxh_c else nkx_f - (nkx_c - i) for i in range(nkx_c)])
+
+
+class Filter:
+    def __init__(self, s_fine, s_coarse):
+        self.rows = coarse_rows(s_fine, s_coarse)
+        self.nky = s_coarse["nky"]
+
+    def __call__(self, F):
+        return np.ascontiguousarray(F[self.rows, :self.nky])
+
+
+def enstrophy_spectrum(om, kx_int, ky_int):
+    """Z(k) = 1/2 sum over the shell round(|k|) = k of |omega|^2, counting the
+    conjugate half plane (ky > 0 twice)."""
+    w = np.where(ky_int[None, :] > 0, 2.0, 1.0)
+    kk = np.rint(np.sqrt(kx_int[:, None] ** 2 + ky_int[None, :] ** 2)).astype(int)
+    return np.bincount(kk.ravel(), weights=(0.5 * w * np.abs(om) ** 2).ravel())
+
+
+def k_ints(s):
+    nkx, nky, nkxh = s["nkx"], s["nky"], s["nkxh"]
+    kx = np.array([i if i < nkxh else i - nkx for i in range(nkx)])
+    return kx, np.arange(nky)
+
+
+class Pipeline:
+    def __init__(self, shrt, cfg):
+        self.shrt, self.cfg = shrt, cfg
+        self.sf = shrt.sizes(cfg.n_fine, cfg.n_fine)
+        self.sc = shrt.sizes(cfg.n_coarse, cfg.n_coarse)
+        self.filt = Filter(self.sf, self.sc)
+        self.net = None
+        self.scale_in = np.ones(3)
+        self.scale_out = np.ones(2)
+
+    def ctx(self, n, sigma, graph_steps=None):
+        gs = self.cfg.interval if graph_steps is None else graph_steps
+        return self.shrt.Shrt(nx=n, ny=n, precision=self.shrt.FP32, graph_steps=gs, **self.cfg.params(sigma))
+
+    def fine_snapshots(self, seed, count):
+        """Filtered (omega, rho, time, shift) of a fine run every `interval` steps."""
+        cfg, S = self.cfg, self.shrt
+        f = self.ctx(cfg.n_fine, cfg.sigma_of(seed))
+        f.initialize_seeded(seed=seed, kmin=cfg.kmin, kmax=cfg.kmax_ic, amp=cfg.amp, expo=cfg.expo)
+        if cfg.spinup:
+            f.time_advance(cfg.spinup)
+        out = []
+        for j in range(count):
+            if j:
+                f.time_advance(cfg.interval)
+            st = f.state()
+            sh, _ = f.shift()
+            out.append((self.filt(f.spectral(S.OMEGA)), self.filt(f.spectral(S.RHO)), st["time"],
+                        np.ascontiguousarray(sh[self.filt.rows]), st["halvings"]))
+        f.close()
+        return out
+
+    def load_state(self, c, om, rh, t, sh):
+        c.set_time(0, t, self.cfg.delt)
+        c.set_shift(sh)
+        c.set_spectral(self.shrt.OMEGA, om)
+        c.set_spectral(self.shrt.RHO, rh)
+
+    def inputs_of(self, c, sigma):
+        S = self.shrt
+        om, rh, ph = c.spectral(S.OMEGA), c.spectral(S.RHO), c.spectral(S.PHI)
+        x = np.empty((4, self.cfg.n_coarse ** 2))
+        for i, F in enumerate((om, rh, ph)):
+            x[i] = c.ktox(F).ravel() / self.scale_in[i]
+        x[3] = sigma
+        return x
+
+    def generate(self, seeds):
+        """Inputs [S][4][n^2] (unscaled) and targets [S][2][n^2]."""
+        cfg, S = self.cfg, self.shrt
+        xs, ys = [], []
+        saved = self.scale_in
+        self.scale_in = np.ones(3)
+        for seed in seeds:
+            sigma = cfg.sigma_of(seed)
+            snaps = self.fine_snapshots(seed, cfg.snaps + 1)
+            c = self.ctx(cfg.n_coarse, sigma)
+            for j in range(cfg.snaps):
+                om, rh, t, sh, _ = snaps[j]
+                self.load_state(c, om, rh, t, sh)
+                x = self.inputs_of(c, sigma)
+                c.time_advance(cfg.interval)
+                dt_tot = cfg.interval * cfg.delt
+                d_om = (snaps[j + 1][0] - c.spectral(S.OMEGA)) / dt_tot
+                d_rh = (snaps[j + 1][1] - c.spectral(S.RHO)) / dt_tot
+                xs.append(x)
+                ys.append(np.stack([c.ktox(d_om).ravel(), c.ktox(d_rh).ravel()]))
+            c.close()
+        self.scale_in = saved
+        return np.array(xs), np.array(ys)
+
+    def fit(self, x, y, iters=2000, batch=8, lr=1e-3, seed=1):
+        """Train the closure network on (x, y); returns the loss history."""
+        cfg = self.cfg
+        self.scale_in = np.maximum(x[:, :3].std(axis=(0, 2)), 1e-30)
+        self.scale_out = np.maximum(y.std(axis=(0, 2)), 1e-30)
+        xn = x.copy()
+        xn[:, :3] /= self.scale_in[None, :, None]
+        yn = y / self.scale_out[None, :, None]
+        if self.net is None:
+            self.net = self.shrt.Closure(cfg.n_coarse, cfg.kmax_net, cfg.width, cfg.layers, max_batch=batch,
+                                         seed=seed)
+        self.net.set_data(xn.astype(np.float32), yn.astype(np.float32))
+        return self.net.train(iters, batch, lr, seed=seed)
+
+    def correction(self, c, sigma):
+        """Spectral corrections of omega and rho (per unit time), or None for the fallback."""
+        out = self.net.forward(self.inputs_of(c, sigma)[None].astype(np.float32))[0].astype(np.float64)
+        if not np.all(np.isfinite(out)):
+            return None
+        n = self.cfg.n_coarse
+        return [c.xtok((out[i] * self.scale_out[i]).reshape(n, n)) for i in range(2)]
+
+    def run_coarse(self, om, rh, t, sh, sigma, steps, closure=False, n=None):
+        """A coarse (or, with n, another plain) run from a filtered state; returns
+        the final (omega, rho) restricted to the coarse modes, the wall time and
+        the number of corrections skipped by the fallback."""
+        cfg, S = self.cfg, self.shrt
+        n = n or cfg.n_coarse
+        c = self.ctx(n, sigma)
+        if n == cfg.n_coarse:
+            self.load_state(c, om, rh, t, sh)
+        else:
+            s_n = self.shrt.sizes(n, n)
+            rows = coarse_rows(s_n, self.sc)
+            Om = np.zeros(c.shape_k, dtype=np.complex64)
+            Rh = np.zeros_like(Om)
+            Om[rows, :self.sc["nky"]] = om
+            Rh[rows, :self.sc["nky"]] = rh
+            # the shift of a row is kx times that of kx = 1 (Lx = 2 pi); the coarse rows keep theirs
+            kxi, _ = k_ints(s_n)
+            shn = kxi * sh[1]
+            shn[rows] = sh
+            self.load_state(c, Om, Rh, t, shn)
+        skipped = 0
+        t0 = time.perf_counter()
+        done = 0
+        while done < steps:
+            k = min(cfg.interval, steps - done)
+            if closure and k == cfg.interval:
+                corr = self.correction(c, sigma)
+                om_c, rh_c = c.spectral(S.OMEGA), c.spectral(S.RHO)
+                dt_tot = cfg.interval * cfg.delt
+                if corr is None or (np.linalg.norm(corr[0]) * dt_tot > cfg.guard * np.linalg.norm(om_c) or
+                                    np.linalg.norm(corr[1]) * dt_tot > cfg.guard * np.linalg.norm(rh_c)):
+                    skipped += 1
+                else:
+                    c.set_spectral(S.OMEGA, om_c + dt_tot * corr[0])
+                    c.set_spectral(S.RHO, rh_c + dt_tot * corr[1])
+            c.time_advance(k)
+            done += k
+        wall = time.perf_counter() - t0
+        if n == cfg.n_coarse:
+            out = (c.spectral(S.OMEGA), c.spectral(S.RHO))
+        else:
+            f = Filter(self.shrt.sizes(n, n), self.sc)
+            out = (f(c.spectral(S.OMEGA)), f(c.spectral(S.RHO)))
+        c.close()
+        return out, wall, skipped
+
+    def errors(self, run, ref):
+        kx, ky = k_ints(self.sc)
+        zr = enstrophy_spectrum(ref[0], kx, ky)
+        zg = enstrophy_spectrum(run[0], kx, ky)
+        m = min(len(zr), self.cfg.kmax_net + 1)
+        ens = float(np.linalg.norm(zg[1:m] - zr[1:m]) / np.linalg.norm(zr[1:m]))
+        rho = float(np.linalg.norm(run[1] - ref[1]) / np.linalg.norm(ref[1]))
+        return dict(enstrophy_spectrum=ens, rho=rho)
+
+    def evaluate(self, seeds, steps):
+        """Held-out seeds: plain coarse, corrected coarse and the finer plain run
+        against the filtered fine run after `steps` steps from the first snapshot."""
+        res = []
+        for seed in seeds:
+            sigma = self.cfg.sigma_of(seed)
+            snaps = self.fine_snapshots(seed, steps // self.cfg.interval + 1)
+            om, rh, t, sh, _ = snaps[0]
+            ref = snaps[-1][:2]
+            plain, w_plain, _ = self.run_coarse(om, rh, t, sh, sigma, steps)
+            corr, w_corr, skipped = self.run_coarse(om, rh, t, sh, sigma, steps, closure=True)
+            mid, w_mid, _ = self.run_coarse(om, rh, t, sh, sigma, steps, n=self.cfg.n_mid)
+            res.append(dict(seed=seed, sigma=sigma, plain=self.errors(plain, ref), closure=self.errors(corr, ref),
+                            mid=self.errors(mid, ref), skipped=skipped,
+                            wall_s=dict(plain=w_plain, closure=w_corr, mid=w_mid)))
+        return res

@@ -43,6 +43,11 @@ constexpr int kDiagBlocks = 512;
 
 }  // namespace
 
+// Error text for entry points defined in other translation units (the closure).
+namespace shrt {
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+}  // namespace shrt
+
 struct shrt_ctx {
     shrt_params p{};
     int prec = 4;

@@ -97,6 +97,16 @@ _SIGS = {
     "shrt_ensemble_step": ([_VP, ctypes.c_int], ctypes.c_int),
     "shrt_ensemble_synchronize": ([_VP], ctypes.c_int),
     "shrt_ensemble_time_steps": ([_VP, ctypes.c_int, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    "shrt_closure_create": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
+                             ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
+    "shrt_closure_destroy": ([_VP], ctypes.c_int),
+    "shrt_closure_info": ([_VP, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "shrt_closure_get_params": ([_VP, _VP], ctypes.c_int),
+    "shrt_closure_set_params": ([_VP, _VP], ctypes.c_int),
+    "shrt_closure_forward": ([_VP, _VP, ctypes.c_int, _VP], ctypes.c_int),
+    "shrt_closure_gradients": ([_VP, _VP, _VP, ctypes.c_int, ctypes.POINTER(ctypes.c_double), _VP], ctypes.c_int),
+    "shrt_closure_set_data": ([_VP, _VP, _VP, ctypes.c_int], ctypes.c_int),
+    "shrt_closure_train": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint64, _VP], ctypes.c_int),
     "shrt_shearing_field": ([_VP, _VP, _VP, _VP], ctypes.c_int),
     "shrt_get_time": ([_VP, ctypes.POINTER(Time)], ctypes.c_int),
     "shrt_set_time": ([_VP, ctypes.c_int64, ctypes.c_double, ctypes.c_double], ctypes.c_int),
@@ -334,6 +344,69 @@ class Ensemble:
     def close(self):
         if getattr(self, "h", None):
             lib().shrt_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Closure:
+    """The learned spectral-convolution closure (shrt_closure_*): inputs
+    [batch][4][n*n] (omega, rho, phi, shear rate), outputs [batch][2][n*n]."""
+
+    def __init__(self, n, kmax, width=32, layers=4, max_batch=8, seed=1):
+        h = ctypes.c_void_p()
+        _check(lib().shrt_closure_create(int(n), int(kmax), int(width), int(layers), int(max_batch), int(seed),
+                                         ctypes.byref(h)))
+        self.h = h
+        self.n, self.kmax, self.width, self.layers, self.max_batch = n, kmax, width, layers, max_batch
+        npar, nm = ctypes.c_longlong(), ctypes.c_int()
+        _check(lib().shrt_closure_info(self.h, ctypes.byref(npar), ctypes.byref(nm)))
+        self.nparams, self.nmodes = npar.value, nm.value
+
+    def params(self):
+        p = np.zeros(self.nparams, dtype=np.float32)
+        _check(lib().shrt_closure_get_params(self.h, _ptr(p)))
+        return p
+
+    def set_params(self, p):
+        a = np.ascontiguousarray(p, dtype=np.float32)
+        assert a.size == self.nparams
+        _check(lib().shrt_closure_set_params(self.h, _ptr(a)))
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        B = x.shape[0]
+        assert x.shape == (B, 4, self.n * self.n)
+        out = np.zeros((B, 2, self.n * self.n), dtype=np.float32)
+        _check(lib().shrt_closure_forward(self.h, _ptr(x), B, _ptr(out)))
+        return out
+
+    def gradients(self, x, target):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        g = np.zeros(self.nparams, dtype=np.float32)
+        loss = ctypes.c_double()
+        _check(lib().shrt_closure_gradients(self.h, _ptr(x), _ptr(t), x.shape[0], ctypes.byref(loss), _ptr(g)))
+        return loss.value, g
+
+    def set_data(self, x, target):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        assert x.shape[0] == t.shape[0]
+        _check(lib().shrt_closure_set_data(self.h, _ptr(x), _ptr(t), x.shape[0]))
+
+    def train(self, iters, batch, lr, seed=0):
+        losses = np.zeros(int(iters))
+        _check(lib().shrt_closure_train(self.h, int(iters), int(batch), float(lr), int(seed), _ptr(losses)))
+        return losses
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().shrt_closure_destroy(self.h)
             self.h = None
 
     def __del__(self):
