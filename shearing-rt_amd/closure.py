@@ -41,6 +41,7 @@ class Config:
         self.spinup = 1000           # fine steps before the first snapshot
         self.snaps = 60              # snapshots per trajectory
         self.nu = 1e-3
+        self.rho0_prime = 1.0        # > 0 unstable (Rayleigh-Taylor), < 0 stable stratification
         self.cfl = 5.0
         self.amp, self.kmin, self.kmax_ic, self.expo = 0.05, 1.0, 4.0, -1.0
         self.sigma_range = (0.5, 1.5)
@@ -53,7 +54,8 @@ class Config:
 
     def params(self, sigma):
         return dict(Lx=TWO_PI, Ly=TWO_PI, delt=self.delt, cfl_num=self.cfl, nu=self.nu, kappa=self.nu,
-                    sigma=float(sigma), g=1.0, rho0=1.0, rho0_prime=1.0, rho_eps1=1e-2, noise=0, linear=0)
+                    sigma=float(sigma), g=1.0, rho0=1.0, rho0_prime=self.rho0_prime, rho_eps1=1e-2, noise=0,
+                    linear=0)
 
     def sigma_of(self, seed):
         lo, hi = self.sigma_range
@@ -161,19 +163,49 @@ class Pipeline:
         self.scale_in = saved
         return np.array(xs), np.array(ys)
 
-    def fit(self, x, y, iters=2000, batch=8, lr=1e-3, seed=1):
-        """Train the closure network on (x, y); returns the loss history."""
+    def normalise(self, x, y=None):
+        xn = x.copy()
+        xn[:, :3] /= self.scale_in[None, :, None]
+        return xn if y is None else (xn, y / self.scale_out[None, :, None])
+
+    def predict(self, xn):
+        """Network outputs (normalised units) for normalised inputs, in batches."""
+        B = self.net.max_batch
+        return np.concatenate([self.net.forward(xn[i:i + B].astype(np.float32)) for i in range(0, len(xn), B)])
+
+    def r2(self, xn, yn):
+        """Fraction of the target variance explained, per output channel."""
+        e = ((self.predict(xn) - yn) ** 2).mean(axis=(0, 2))
+        return 1.0 - e / (yn ** 2).mean(axis=(0, 2))
+
+    def fit(self, x, y, iters=2000, batch=8, lr=1e-3, seed=1, val=None, chunk=250):
+        """Train the closure network on (x, y); returns the loss history.  With a
+        validation set val = (xv, yv) training runs in chunks and keeps the
+        parameters with the best mean validation R^2 (early stopping)."""
         cfg = self.cfg
         self.scale_in = np.maximum(x[:, :3].std(axis=(0, 2)), 1e-30)
         self.scale_out = np.maximum(y.std(axis=(0, 2)), 1e-30)
-        xn = x.copy()
-        xn[:, :3] /= self.scale_in[None, :, None]
-        yn = y / self.scale_out[None, :, None]
+        xn, yn = self.normalise(x, y)
         if self.net is None:
             self.net = self.shrt.Closure(cfg.n_coarse, cfg.kmax_net, cfg.width, cfg.layers, max_batch=batch,
                                          seed=seed)
         self.net.set_data(xn.astype(np.float32), yn.astype(np.float32))
-        return self.net.train(iters, batch, lr, seed=seed)
+        if val is None:
+            return self.net.train(iters, batch, lr, seed=seed)
+        xv, yv = self.normalise(*val)
+        best, best_p, losses, self.val_curve = -np.inf, None, [], []
+        done = 0
+        while done < iters:
+            k = min(chunk, iters - done)
+            losses.append(self.net.train(k, batch, lr, seed=seed + done))
+            done += k
+            score = float(np.mean(self.r2(xv, yv)))
+            self.val_curve.append((done, score))
+            if score > best:
+                best, best_p = score, self.net.params()
+        self.net.set_params(best_p)
+        self.val_best = best
+        return np.concatenate(losses)
 
     def correction(self, c, sigma):
         """Spectral corrections of omega and rho (per unit time), or None for the fallback."""
