@@ -151,6 +151,46 @@ def run_ensemble(shrt, args):
     print(json.dumps(out), flush=True)
 
 
+CLOSURE_DEFAULT = dict(n_fine=256, n_coarse=64, n_mid=128, delt=1e-3, interval=20, spinup=1000, snaps=40,
+                       nu=1.25e-4, rho0_prime=-1.0, amp=0.5, kmin=1.0, kmax_ic=16.0, kmax_net=16, width=16, layers=4)
+
+
+def run_closure(shrt, args):
+    """Learned closure versus plain solvers: data from fine runs, device
+    training with early stopping, then held-out seeds judged against the
+    filtered fine run (enstrophy spectrum and rho relative L2)."""
+    cl = _load("closure_pipe", os.path.join(REPO, "shearing-rt_amd", "closure.py"))
+    kw = dict(CLOSURE_DEFAULT)
+    kw.update(json.loads(args.closure))
+    ntrain, nval = kw.pop("trajectories", 64), kw.pop("val_trajectories", 4)
+    iters, lr, batch = kw.pop("iters", 2000), kw.pop("lr", 5e-4), kw.pop("batch", 16)
+    tests, evsteps = kw.pop("test_seeds", [900, 901, 902, 903]), kw.pop("eval_steps", 1000)
+    cfg = cl.Config(**kw)
+    p = cl.Pipeline(shrt, cfg)
+    t0 = time.perf_counter()
+    x, y = p.generate(range(100, 100 + ntrain))
+    val = p.generate(range(800, 800 + nval))
+    t_data = time.perf_counter() - t0
+    log(f"closure: {len(x)} training samples in {t_data:.1f} s")
+    t0 = time.perf_counter()
+    losses = p.fit(x, y, iters=iters, batch=batch, lr=lr, val=val)
+    t_train = time.perf_counter() - t0
+    log(f"closure: trained {iters} iterations in {t_train:.1f} s, best validation R2 {p.val_best:.3f}")
+    res = p.evaluate(tests, evsteps)
+    mean = lambda run, key: float(np.mean([r[run][key] for r in res]))
+    wall = lambda run: float(np.mean([r["wall_s"][run] for r in res]))
+    out = {"metric": "learned closure: enstrophy-spectrum / rho relative L2 error vs filtered fine run",
+           "config": dict(kw, trajectories=ntrain, iters=iters, eval_steps=evsteps, test_seeds=tests),
+           "samples": int(len(x)), "data_s": t_data, "train_s": t_train, "train_loss_last": float(losses[-1]),
+           "val_r2_best": p.val_best, "val_curve": p.val_curve,
+           "error": {run: {"enstrophy_spectrum": mean(run, "enstrophy_spectrum"), "rho": mean(run, "rho")}
+                     for run in ("plain", "closure", "mid")},
+           "wall_s_per_run": {run: wall(run) for run in ("plain", "closure", "mid")},
+           "skipped_corrections": int(sum(r["skipped"] for r in res)), "per_seed": res}
+    p.net.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -173,6 +213,9 @@ def main():
     ap.add_argument("--autotune-report", default="", metavar="SIZES",
                     help="only run shrt_autotune for each square size (e.g. 256,512,1024,2048,4096) and "
                          "print the per-variant kernel times as JSON lines, then exit")
+    ap.add_argument("--closure", default="", metavar="JSON",
+                    help="only run the learned-closure experiment (JSON overrides of its configuration, e.g. '{}') "
+                         "and exit")
     ap.add_argument("--ensemble", default="", metavar="SPEC",
                     help="only time an ensemble, e.g. 128x24,256x24,512x16 (grid x count; FP32, mixed "
                          "shear rates and seeds, one captured graph per --graph-steps steps) and exit")
@@ -208,6 +251,9 @@ def main():
 
     if args.ensemble:
         run_ensemble(shrt, args)
+        return
+    if args.closure:
+        run_closure(shrt, args)
         return
     if args.autotune_report:
         prec = shrt.FP32 if args.precision == "fp32" else shrt.FP64
