@@ -94,7 +94,7 @@ def pmc_summary(fetch_csv, write_csv, cal_fetch_csv, cal_write_csv, cal_bytes):
     w_cal = cal_bytes / (copy_kib(per_kernel(cal_write_csv)) * 1024.0)
     fetch, write = per_kernel(fetch_csv), per_kernel(write_csv)
     out = {"calibration": {"copy_bytes": cal_bytes, "fetch_factor": f_cal, "write_factor": w_cal}}
-    pat = re.compile(r"k_(p[123])(?:_row)?<(float|double), (\d+)>")
+    pat = re.compile(r"k_(p[123])(?:_row|_pair)?<(float|double), (\d+)>")
     for k, fv in fetch.items():
         m = pat.search(k)
         if not m or k not in write:
@@ -200,6 +200,8 @@ def main():
     ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
     ap.add_argument("--graph-steps", type=int, default=20)
     ap.add_argument("--kernel-steps", type=int, default=50)
+    ap.add_argument("--variants", default="", metavar="P1,P2,P3",
+                    help="kernel variants of the three passes (shrt_set_variants); default: the library's")
     ap.add_argument("--fusion", type=int, choices=[0, 1], default=0,
                     help="captured batches fuse P3 of step n with P1 of step n+1 (1) or not (0, default)")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU-baseline work")
@@ -262,7 +264,7 @@ def main():
                           cfl_num=0.35, nu=1e-4, kappa=1e-4, sigma=1.0, g=1.0, rho0=1.0, rho0_prime=1.0,
                           rho_eps1=1e-2, noise=0, linear=0)
             t = g.autotune(reps=20)
-            names = ["p1_row", "p1_tile", "p2_three", "p2_four", "p3_pair", "p3_tile"]
+            names = ["p1_row", "p1_tile", "p1_pair", "p2_three", "p2_four", "p3_pair", "p3_tile"]
             print(json.dumps({"n": n, "precision": args.precision, "choice": [t["p1"], t["p2"], t["p3"]],
                               "us_per_launch": {k: round(v * 1e3, 2) for k, v in zip(names, t["ms"])},
                               "fuse": t["fuse"], "us_per_step": {"unfused": round(t["step_ms"][0] * 1e3, 2),
@@ -280,6 +282,8 @@ def main():
     log(f"bench: {n}^2 {args.precision}, graph_steps={args.graph_steps}, warmup={args.warmup}, steps={args.steps}")
     g = shrt.Shrt(nx=n, ny=n, precision=prec, graph_steps=args.graph_steps, **params)
     g.set_fusion(args.fusion)
+    if args.variants:
+        g.set_variants(*[int(v) for v in args.variants.split(",")])
     g.initialize_seeded(**ic)
     if args.warmup > 0:
         g.time_advance(args.warmup)
@@ -322,7 +326,8 @@ def main():
         "data": "synthetic: omega=0, seeded random-phase rho on 2<=|k|<=32, amplitude 1e-4|k|^-1.5",
         "config": {"workload": f"pseudospectral step, {n}^2 {args.precision.upper()} single instance "
                                "(sigma=1, nu=kappa=1e-4, g=1, delt=1e-4)",
-                   "nx": n, "ny": n, "graph_steps": args.graph_steps, "fusion": args.fusion},
+                   "nx": n, "ny": n, "graph_steps": args.graph_steps, "fusion": args.fusion,
+                   "variants": args.variants or "default"},
         "roofline": {"bound": "hbm", "kernel": dname, "achieved": achieved, "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
                      "bytes_per_launch": kb[dname], "ms_per_launch": float(kt[dom])},

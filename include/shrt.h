@@ -189,20 +189,21 @@ shrt_status shrt_kernel_times(shrt_ctx* ctx, int nsteps, double* ms3);
  * reference point that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings. */
 shrt_status shrt_copy_bench(long long bytes, int reps, double* ms);
 
-/* Autotuning (new; no reference counterpart).  Each pass of the step has two
- * kernel variants: P1 one (row pair, output) per workgroup (0) or a tile of four
- * row pairs (1); P2 three x-inverse transforms per column (0) or four (1); P3
- * one row pair per workgroup (0) or four (1).  shrt_autotune times both
- * variants of each pass on a scratch instance of the context's shape and
- * parameters (reps launches each) and keeps the faster.  The variants agree
- * to rounding (the P2 variants pair the fields differently in the complex
- * transforms; all are compiled separately).  ms[] = average ms per launch of
- * p1 row, p1 tile, p2 three, p2 four, p3 pair, p3 tile.  It then times
+/* Autotuning (new; no reference counterpart).  Each pass of the step has
+ * kernel variants: P1 one (row pair, output) per workgroup (0), a tile of four
+ * row pairs (1) or one (row pair, field group) with its two outputs (2); P2
+ * three x-inverse transforms per column (0) or four (1); P3 one row pair per
+ * workgroup (0) or four (1).  shrt_autotune times the variants of each pass on
+ * a scratch instance of the context's shape and parameters (reps launches
+ * each) and keeps the fastest.  The variants agree to rounding (the P2
+ * variants pair the fields differently in the complex transforms; all are
+ * compiled separately).  ms[] = average ms per launch of p1 row, p1 tile,
+ * p1 pair, p2 three, p2 four, p3 pair, p3 tile.  It then times
  * captured batches of graph_steps steps (20 if the context has none) with and
  * without the P3+P1 fusion (shrt_set_fusion) and keeps the faster. */
 typedef struct {
     int p1, p2, p3;
-    double ms[6];
+    double ms[7];
     int fuse;            /* 1: captured batches fuse P3 with the next P1 (shrt_set_fusion) */
     double step_ms[2];   /* ms per step of captured batches, unfused and fused */
 } shrt_tuning;

@@ -709,14 +709,16 @@ shrt_status shrt_autotune(shrt_ctx* c, int reps, shrt_tuning* out) {
     shrt_seeded_ic ic{12345u, 2.0, std::max(2.0, c->g.ny / 16.0), 1e-4, -1.5};
     st = shrt_initialize_seeded(t, &ic);
     if (st == SHRT_OK) st = shrt_step(t, 4);
-    static const int nvar[3] = {2, 2, 2};
-    double ms[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+    static const int nvar[3] = {3, 2, 2};
+    double ms[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     for (int pass = 0; pass < 3 && st == SHRT_OK; ++pass)
         for (int v = 0; v < nvar[pass] && st == SHRT_OK; ++v)
             st = c->prec == 4 ? time_pass<float>(t, pass, v, reps, &ms[pass][v])
                               : time_pass<double>(t, pass, v, reps, &ms[pass][v]);
     Variants chosen;
-    chosen.p1 = ms[0][1] < ms[0][0] ? kP1Tile : kP1Row;
+    chosen.p1 = kP1Row;
+    for (int v = 1; v < nvar[0]; ++v)
+        if (ms[0][v] < ms[0][chosen.p1]) chosen.p1 = v;
     chosen.p2 = ms[1][1] < ms[1][0] ? kP2Four : kP2Three;
     chosen.p3 = ms[2][1] < ms[2][0] ? kP3Tile : kP3Pair;
     // captured batches with and without the P3+P1 fusion, alternated, best of two each
@@ -744,8 +746,9 @@ shrt_status shrt_autotune(shrt_ctx* c, int reps, shrt_tuning* out) {
         out->p1 = c->var.p1;
         out->p2 = c->var.p2;
         out->p3 = c->var.p3;
+        int k = 0;
         for (int pass = 0; pass < 3; ++pass)
-            for (int v = 0; v < 2; ++v) out->ms[2 * pass + v] = ms[pass][v];
+            for (int v = 0; v < nvar[pass]; ++v) out->ms[k++] = ms[pass][v];
         out->fuse = c->var.fuse;
         out->step_ms[0] = step_ms[0];
         out->step_ms[1] = step_ms[1];
@@ -754,7 +757,7 @@ shrt_status shrt_autotune(shrt_ctx* c, int reps, shrt_tuning* out) {
 }
 
 shrt_status shrt_set_variants(shrt_ctx* c, int p1, int p2, int p3) {
-    if (!c || p1 < 0 || p1 > 1 || p2 < 0 || p2 > 1 || p3 < 0 || p3 > 1) return fail(SHRT_ERR_INVALID, "bad variant");
+    if (!c || p1 < 0 || p1 > 2 || p2 < 0 || p2 > 1 || p3 < 0 || p3 > 1) return fail(SHRT_ERR_INVALID, "bad variant");
     c->var.p1 = p1;
     c->var.p2 = p2;
     c->var.p3 = p3;

@@ -71,7 +71,11 @@ template <>
 hipError_t Launch<R>::p1(const StepArgs<R>& a, const StepArgs<R>* dargs, int count, int variant, hipStream_t s) {
     bool ok = dispatch(a.g.ny, [&](auto nc) {
         constexpr int N = decltype(nc)::value;
-        if (variant == kP1Row) {
+        if (variant == kP1Pair) {
+            const dim3 grid(a.g.nkxh * (a.ph.linear ? 1 : 3), dargs ? count : 1);
+            if (dargs) hipLaunchKernelGGL((k_p1_pair<R, N, true>), grid, dim3(TeamShape<N>::TT), 0, s, a, dargs);
+            else hipLaunchKernelGGL((k_p1_pair<R, N, false>), grid, dim3(TeamShape<N>::TT), 0, s, a, dargs);
+        } else if (variant == kP1Row) {
             const dim3 grid((a.g.nkxh - 1) * (a.ph.linear ? 2 : 6) + (a.ph.linear ? 1 : 3), dargs ? count : 1);
             if (dargs) hipLaunchKernelGGL((k_p1_row<R, N, true>), grid, dim3(TeamShape<N>::TT), 0, s, a, dargs);
             else hipLaunchKernelGGL((k_p1_row<R, N, false>), grid, dim3(TeamShape<N>::TT), 0, s, a, dargs);

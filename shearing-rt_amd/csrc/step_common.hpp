@@ -43,7 +43,8 @@ template <typename T, int N> constexpr int p3_points() { return TeamShape<N>::P;
 // pass.  Defaults: the fastest at 2048^2 FP32 (P1 31.3 us row against 32.4 us
 // tile; P2 34.4 us three transforms against 36.8 us four; P3 29.7 us one row
 // pair against 32.0 us four).
-enum : int { kP1Row = 0, kP1Tile = 1 };        // one (row pair, output) / a tile of kTile row pairs per workgroup
+enum : int { kP1Row = 0, kP1Tile = 1, kP1Pair = 2 };   // per workgroup: one (row pair, output) / a tile of kTile
+                                                       // row pairs / one (row pair, field group): two outputs
 enum : int { kP2Three = 0, kP2Four = 1 };      // x-inverse transforms per column
 enum : int { kP3Pair = 0, kP3Tile = 1 };       // one row pair / kTile row pairs per workgroup
 struct Variants {

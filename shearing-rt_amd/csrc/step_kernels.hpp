@@ -290,6 +290,123 @@ k_p1_row(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
     }
 }
 
+// The y-inverse transforms of outputs [o_begin, o_end) of the row pair
+// (row, -row) from its spectrum before multipliers: base[m] for element
+// e = tid + m*TT (row kx at iky = e, or row -kx at iky = NY - e, not yet
+// conjugated; zero in the padding), base_n0 = the mirror row's ky = 0 value,
+// with the shifts sn[] and wavenumbers kxr[] of the two rows; written to G at
+// the row's position, and aphi = omega/kperp2_shear (shear.cu:367-374) when
+// store_phi.  One transform per output in a rolled loop whose lane index is
+// opaque, so the per-mode wavenumbers are recomputed per output rather than
+// hoisted and kept live (48 more registers) across the transforms.  Output oi
+// multiplies X = phi (oi < 2) or the field by i*c: c = kys (0, 3, 5), -kx (1),
+// kx (2, 4) -- p1_mult in selected form.
+template <typename T, int NY>
+__device__ __forceinline__ void p1_outputs(const StepArgs<T>& a, int row, const cplx<T>* base, cplx<T> base_n0,
+                                           const double* sn, const double* kxr, int o_begin, int o_end,
+                                           int store_phi, cplx<T>* lds, int tid) {
+    using S = TeamShape<NY>;
+    constexpr int P = S::P, TT = S::TT;
+    constexpr int NKY = NY / 3 + 1;
+    const Geo& g = a.g;
+    const int rn = row == 0 ? 0 : g.nkx - row;
+    const int gp = g_pos_of(row, g.nkxh);
+    const TeamSync<S::TT> sync{};
+#pragma nounroll
+    for (int oi = o_begin; oi < o_end; ++oi) {
+        int tl = tid;
+        asm volatile("" : "+v"(tl));
+        tl &= TT - 1;
+        const bool use_phi = oi < 2, use_kys = oi == 0 || oi == 3 || oi == 5, neg = oi == 1;
+        auto mult = [&](const ModeK<T>& k, bool origin, cplx<T> F) -> cplx<T> {
+            if (use_phi) F = origin ? czero<T>() : cdivr(F, k.k2);
+            const T cf = use_kys ? k.kys : (neg ? -k.kx : k.kx);
+            return {-cf * F.y, cf * F.x};
+        };
+        cplx<T> v[P];
+#pragma unroll
+        for (int m = 0; m < P; ++m) {
+            const int e = tl + m * TT;
+            const bool pos = e < NKY;
+            const bool keep = pos || e > NY - NKY;
+            const int iky = keep ? (pos ? e : NY - e) : 0;
+            const int side = pos ? 0 : 1;
+            const int r = side ? rn : row;
+            const ModeK<T> k = mode_k<T>(a.ky, sn[side], kxr[side], iky);
+            const cplx<T> z = mult(k, r == 0 && iky == 0, base[m]);
+            v[m] = keep ? (pos ? z : cplx<T>{z.x, -z.y}) : czero<T>();
+        }
+        if (tl == 0) {   // Hermitian part of the ky = 0 column
+            const ModeK<T> k = mode_k<T>(a.ky, sn[1], kxr[1], 0);
+            const cplx<T> vn = mult(k, rn == 0, base_n0);
+            v[0] = {(v[0].x + vn.x) * T(0.5), (v[0].y - vn.y) * T(0.5)};
+        }
+        if (oi == 0 && store_phi) {
+#pragma unroll
+            for (int m = 0; m < P; ++m) {
+                const int e = tl + m * TT;
+                const bool pos = e < NKY;
+                if (!(pos || e > NY - NKY) || (!pos && row == 0)) continue;
+                const int iky = pos ? e : NY - e;
+                const int side = pos ? 0 : 1;
+                const int r = side ? rn : row;
+                const ModeK<T> k = mode_k<T>(a.ky, sn[side], kxr[side], iky);
+                a.aphi[(long long)r * g.nky + iky] = (r == 0 && iky == 0) ? czero<T>() : cdivr(base[m], k.k2);
+            }
+            if (tl == 0 && row != 0) {
+                const ModeK<T> k = mode_k<T>(a.ky, sn[1], kxr[1], 0);
+                a.aphi[(long long)rn * g.nky] = cdivr(base_n0, k.k2);
+            }
+        }
+        TeamFFT<T, NY, P, S::RMAX, +1>::run(v, lds, a.twy, tid, sync);
+        cplx<T>* Gf = a.G + (long long)oi * g.I;
+#pragma unroll
+        for (int m = 0; m < P; ++m) {
+            const int y = tl + m * TT;
+            Gf[((long long)(y >> 2) * g.nkxh + gp) * 4 + (y & 3)] = v[m];
+        }
+    }
+}
+
+// P1 with one (row pair, field group) per workgroup: the row pair of omega
+// (groups 0: vx, vy and 1: d/dx, d/dy omega) or rho (group 2) is read once
+// through the remap view and both outputs of the group are transformed from
+// it; 3 * nkxh workgroups (nkxh in the linear mode, group 0 only).
+template <typename T, int NY, bool BATCH>
+__global__ void __launch_bounds__(TeamShape<NY>::TT, (waves_per_eu<T, NY, TeamShape<NY>::TT, 1>()))
+k_p1_pair(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
+    const StepArgs<T>& a = step_args_of<BATCH>(a1, args);
+    using S = TeamShape<NY>;
+    constexpr int P = S::P, TT = S::TT, L = lds_len<NY>();
+    constexpr int NKY = NY / 3 + 1;
+    __shared__ cplx<T> sm[L];
+    const Geo& g = a.g;
+    const int tid = threadIdx.x;
+    const int ngrp = a.ph.linear ? 1 : 3;
+    const int q = xcd_item(blockIdx.x, gridDim.x);
+    const int pos = q / ngrp, grp = q - pos * ngrp;
+    const int row = p1_row_of(pos, g.nkxh);
+    const int rn = row == 0 ? 0 : g.nkx - row;
+    const StepCtl* c = a.ctl_in;
+    const int s1 = slot_a1(c->istep);
+    const cplx<T>* src = grp == 2 ? a.a_rho[s1] : a.a_omg[s1];
+    const int* jmp = (c->slot_epoch_a[s1] != c->epoch) ? a.jump + (c->epoch & 1) * g.nkx : nullptr;
+    const double sn[2] = {a.s_in[row], a.s_in[rn]};
+    const double kxr[2] = {a.kx[row], a.kx[rn]};
+    cplx<T> base[P];
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+        const int e = tid + m * TT;
+        const bool p0 = e < NKY;
+        const bool keep = p0 || e > NY - NKY;
+        const int iky = keep ? (p0 ? e : NY - e) : 0;
+        const cplx<T> F = view_load(src, p0 ? row : rn, iky, g, jmp);
+        base[m] = keep ? F : czero<T>();
+    }
+    const cplx<T> base_n0 = view_load(src, rn, 0, g, jmp);
+    p1_outputs<T, NY>(a, row, base, base_n0, sn, kxr, 2 * grp, 2 * grp + 2, a.store_phi, sm, tid);
+}
+
 // ---------------------------------------------------------------------------
 // P2
 // ---------------------------------------------------------------------------
@@ -844,67 +961,8 @@ k_p3(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
         }
         const cplx<T> base_n0 = viewval(1, 0);   // the mirror row's ky = 0 value
         __syncthreads();   // LDS is the transforms' scratch from here on
-        const int gp = g_pos_of(row, g.nkxh);
-        // One transform per output, in a rolled loop whose lane index is opaque
-        // so that the per-mode wavenumbers are recomputed per output rather
-        // than hoisted and kept live (48 more registers) across the transforms.
-        // Output oi multiplies X = phi (oi < 2) or the field by i*c:
-        // c = kys (0, 3, 5), -kx (1), kx (2, 4)  -- p1_mult in selected form.
         const int o_end = f == 0 ? (ph.linear ? 2 : 4) : 6;
-#pragma nounroll
-        for (int oi = f == 0 ? 0 : 4; oi < o_end; ++oi) {
-            int tl = tid;
-            asm volatile("" : "+v"(tl));
-            tl &= TT - 1;
-            const bool use_phi = oi < 2, use_kys = oi == 0 || oi == 3 || oi == 5, neg = oi == 1;
-            auto mult = [&](const ModeK<T>& k, bool origin, cplx<T> F) -> cplx<T> {
-                if (use_phi) F = origin ? czero<T>() : cdivr(F, k.k2);
-                const T cf = use_kys ? k.kys : (neg ? -k.kx : k.kx);
-                return {-cf * F.y, cf * F.x};
-            };
-            cplx<T> v[P];
-#pragma unroll
-            for (int m = 0; m < P; ++m) {
-                const int e = tl + m * TT;
-                const bool pos = e < NKY;
-                const bool keep = pos || e > NY - NKY;
-                const int iky = keep ? (pos ? e : NY - e) : 0;
-                const int side = pos ? 0 : 1;
-                const int r = side ? rn : row;
-                const ModeK<T> k = mode_k<T>(a.ky, sn[side], kx_r[side], iky);
-                const cplx<T> z = mult(k, r == 0 && iky == 0, base[m]);
-                v[m] = keep ? (pos ? z : cplx<T>{z.x, -z.y}) : czero<T>();
-            }
-            if (tl == 0) {   // Hermitian part of the ky = 0 column
-                const ModeK<T> k = mode_k<T>(a.ky, sn[1], kx_r[1], 0);
-                const cplx<T> vn = mult(k, rn == 0, base_n0);
-                v[0] = {(v[0].x + vn.x) * T(0.5), (v[0].y - vn.y) * T(0.5)};
-            }
-            if (oi == 0 && a.store_phi_next) {   // aphi = omega/kperp2_shear of step n+1 (shear.cu:367-374)
-#pragma unroll
-                for (int m = 0; m < P; ++m) {
-                    const int e = tl + m * TT;
-                    const bool pos = e < NKY;
-                    if (!(pos || e > NY - NKY) || (!pos && row == 0)) continue;
-                    const int iky = pos ? e : NY - e;
-                    const int side = pos ? 0 : 1;
-                    const int r = side ? rn : row;
-                    const ModeK<T> k = mode_k<T>(a.ky, sn[side], kx_r[side], iky);
-                    a.aphi[(long long)r * g.nky + iky] = (r == 0 && iky == 0) ? czero<T>() : cdivr(base[m], k.k2);
-                }
-                if (tl == 0 && row != 0) {
-                    const ModeK<T> k = mode_k<T>(a.ky, sn[1], kx_r[1], 0);
-                    a.aphi[(long long)rn * g.nky] = cdivr(base_n0, k.k2);
-                }
-            }
-            TeamFFT<T, NY, P, S::RMAX, +1>::run(v, lds, a.twy, tid, sync);
-            cplx<T>* Gf = a.G + (long long)oi * g.I;
-#pragma unroll
-            for (int m = 0; m < P; ++m) {
-                const int y = tl + m * TT;
-                Gf[((long long)(y >> 2) * g.nkxh + gp) * 4 + (y & 3)] = v[m];
-            }
-        }
+        p1_outputs<T, NY>(a, row, base, base_n0, sn, kx_r, f == 0 ? 0 : 4, o_end, a.store_phi_next, lds, tid);
     }
 }
 

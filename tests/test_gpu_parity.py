@@ -201,9 +201,10 @@ def test_fused_batches_with_remaps(gpu, prec, nx, ny, linear):
     so = o.state()
     assert so["remap_count"] == 4 and so["last_remap_step"] == 19
     outs = {}
-    for mode in ("fused", "unfused", "single"):
+    for mode in ("fused", "unfused", "single", "p1pair"):
         g = shrt.Shrt(nx=nx, ny=ny, precision=prec, graph_steps=0 if mode == "single" else 4, **dict(BASE, **p))
         g.set_fusion(mode == "fused")
+        g.set_variants(2 if mode == "p1pair" else 0, 0, 0)
         g.initialize_seeded(**ic)
         g.time_advance(steps)
         sg = g.state()
@@ -218,7 +219,7 @@ def test_fused_batches_with_remaps(gpu, prec, nx, ny, linear):
             print(f"  {mode} {ring} rel err {err:.3e}", flush=True)
             assert err < TOLN[prec], (mode, ring, err)
         g.close()
-    for mode in ("unfused", "single"):
+    for mode in ("unfused", "single", "p1pair"):
         for a, b in zip(outs["fused"], outs[mode]):
             same = np.array_equal(a.view(np.uint8), b.view(np.uint8))
             print(f"  fused vs {mode}: {'bit-identical' if same else f'rel {rel_l2(a, b):.2e}'}")
@@ -373,7 +374,7 @@ def test_kernel_variants(gpu, prec):
     o.step(steps)
     ref = o.buffer("arho")
     outs = {}
-    for v in [(a, b, c) for a in (0, 1) for b in (0, 1) for c in (0, 1)]:
+    for v in [(a, b, c) for a in (0, 1, 2) for b in (0, 1) for c in (0, 1)]:
         g = shrt.Shrt(nx=n, ny=n, precision=prec, graph_steps=4, **dict(BASE, sigma=1.0))
         g.set_variants(*v)
         g.initialize_seeded(**ic)
@@ -385,7 +386,7 @@ def test_kernel_variants(gpu, prec):
         assert rel_l2(r, outs[(0, 0, 0)]) < TOLN[prec], v
     g = shrt.Shrt(nx=n, ny=n, precision=prec, graph_steps=4, **dict(BASE, sigma=1.0))
     t = g.autotune(reps=5)
-    assert t["p1"] in (0, 1) and t["p2"] in (0, 1) and t["p3"] in (0, 1) and t["fuse"] in (0, 1)
+    assert t["p1"] in (0, 1, 2) and t["p2"] in (0, 1) and t["p3"] in (0, 1) and t["fuse"] in (0, 1)
     assert all(m > 0 for m in t["ms"]) and all(m > 0 for m in t["step_ms"])
     assert t["fuse"] == (t["step_ms"][1] < t["step_ms"][0])
     g.initialize_seeded(**ic)     # the tuned context steps (fused or not) and matches the oracle
