@@ -654,6 +654,50 @@ __device__ __forceinline__ cplx<T> ab_bdf_update(int ist, bool plain, double dt,
     return r;
 }
 
+// FP32 contexts: the state update (AB/BDF2 combination and implicit
+// diffusion) is evaluated in FP64 from the FP32 operands and rounded once, so
+// each step adds one rounding of the state instead of about five; the
+// tendency, which enters multiplied by the step, stays FP32.
+constexpr bool kWideUpdate = true;
+
+// The same update for an FP32 state evaluated in FP64 and rounded once: the
+// divisions by 3 and by the diffusion denominator become one reciprocal and
+// products (their FP64 rounding is far below the final FP32 rounding).
+__device__ __forceinline__ cplx<float> ab_bdf_update_wide(int ist, bool plain, double dt, double diff, double k2,
+                                                         cplx<float> a1f, cplx<float> a2f, cplx<float> d0f,
+                                                         cplx<float> d1f, cplx<float> d2f) {
+    const cplx<double> a1 = {a1f.x, a1f.y}, a2 = {a2f.x, a2f.y};
+    const cplx<double> d0 = {d0f.x, d0f.y}, d1 = {d1f.x, d1f.y}, d2 = {d2f.x, d2f.y};
+    constexpr double third = 1.0 / 3.0;
+    cplx<double> r;
+    if (ist == 0) {
+        r = {a1.x + dt * d0.x, a1.y + dt * d0.y};
+    } else if (ist == 1) {
+        if (plain) {
+            r = {a1.x + dt * (1.5 * d0.x - 0.5 * d1.x), a1.y + dt * (1.5 * d0.y - 0.5 * d1.y)};
+        } else {
+            const double c = (2.0 / 3.0) * dt;
+            r = {(4.0 * a1.x - a2.x) * third + c * (2.0 * d0.x - d1.x),
+                 (4.0 * a1.y - a2.y) * third + c * (2.0 * d0.y - d1.y)};
+        }
+    } else {
+        if (plain) {
+            const double c = dt / 12.0;
+            r = {a1.x + c * ((23.0 * d0.x - 16.0 * d1.x) + 5.0 * d2.x), a1.y + c * ((23.0 * d0.y - 16.0 * d1.y) + 5.0 * d2.y)};
+        } else {
+            const double c = dt / 9.0;
+            r = {(4.0 * a1.x - a2.x) * third + c * ((16.0 * d0.x - 14.0 * d1.x) + 4.0 * d2.x),
+                 (4.0 * a1.y - a2.y) * third + c * ((16.0 * d0.y - 14.0 * d1.y) + 4.0 * d2.y)};
+        }
+    }
+    if (!plain) {
+        const double cd = ist == 0 ? dt : (2.0 / 3.0) * dt;
+        const double inv = 1.0 / (1.0 + cd * diff * k2);
+        r = {r.x * inv, r.y * inv};
+    }
+    return {(float)r.x, (float)r.y};
+}
+
 // Bookkeeping of step n done by workgroup 0 of P3: the shift advance and remap
 // trigger of update_shear (shear.cu:211-294) in FP64, and the next control block.
 template <typename T>
@@ -840,15 +884,20 @@ k_p3(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
                 nh = {F.x * scale, F.y * scale};
             }
             const double k2 = k2_of(kxv, __dadd_rn(B.ky[u], sr));
-            cplx<T> d0;
+            cplx<T> d0, an;
             if (f == 0) {
                 d0 = {nh.x + cc * B.x1[u].y, nh.y - cc * B.x1[u].x};
             } else {   // phi = omega/kperp2 (zero at the origin)
                 const cplx<T> phi = (r == 0 && iky == 0) ? czero<T>() : cdivr(B.x1[u], (T)k2);
                 d0 = {nh.x - cc * phi.y, nh.y + cc * phi.x};
             }
+            if constexpr (sizeof(T) == 4 && kWideUpdate) {
+                // FP32 state: the AB/BDF2 combination and the implicit diffusion in FP64, rounded once
+                an = ab_bdf_update_wide(ist, plain, dt, diff, k2, B.a1[u], B.a2[u], d0, B.d1[u], B.d2[u]);
+            } else {
+                an = ab_bdf_update<T>(ist, plain, dt, diff, k2, B.a1[u], B.a2[u], d0, B.d1[u], B.d2[u]);
+            }
             const long long at = (long long)r * g.nky + iky;
-            const cplx<T> an = ab_bdf_update<T>(ist, plain, dt, diff, k2, B.a1[u], B.a2[u], d0, B.d1[u], B.d2[u]);
             A0[at] = an;
             D0[at] = d0;
             if constexpr (FUSE) {

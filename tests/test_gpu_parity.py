@@ -140,6 +140,75 @@ def test_multistep_sizes(gpu, prec, nx, ny, steps):
 
 
 @pytest.mark.parametrize("prec", [8, 4])
+def test_small_case_full_length(gpu, prec):
+    """The plain small case at its full length: 128^2, Lx = Ly = 2 pi, nu = kappa
+    = 1e-3, g = 1, shear 0.5 (a remap whenever 0.5 t crosses an integer), dt =
+    1e-3, 2000 steps (Euler and AB2 start-up, then AB3/BDF2), omega = 0 and a
+    seeded random-phase rho on 2 <= |k| <= 8 with amplitude 1e-3 |k|^-1.  Fields
+    within 1e-10 (FP64) and 1e-5 (FP32) relative L2 of the FP64 oracle after
+    1000 steps; after 2000 (the remap falls on the last step) FP64 still within
+    1e-10 and FP32 within 3e-5 (the FP32 state is rounded every step, so its
+    error keeps growing); the shift bit-exact; energy, enstrophy and rho
+    variance of the final state within the same tolerances.  Captured batches of
+    20 steps, as the driver runs them."""
+    n = 128
+    print(f"small case full length prec={prec}", flush=True)
+    p = dict(BASE, sigma=0.5, nu=1e-3, kappa=1e-3, delt=1e-3)
+    o = oracle.Oracle(threads=8, nx=n, ny=n, **p)
+    g = shrt.Shrt(nx=n, ny=n, precision=prec, graph_steps=20, **p)
+    ic = dict(seed=20240601, kmin=2.0, kmax=8.0, amp=1e-3, expo=-1.0)
+    o.set_spectral("rho", oracle.seeded_spectrum(n, n, 2 * np.pi, 2 * np.pi, **ic))
+    g.initialize_seeded(**ic)
+    tol = TOLN[prec]
+    for steps in (1000, 2000):
+        o.step(1000)
+        g.time_advance(1000)
+        for f, ring in ((shrt.OMEGA, "aomg"), (shrt.RHO, "arho"), (shrt.PHI, "aphi")):
+            err = rel_l2(g.spectral(f), o.buffer(ring))
+            print(f"  step {steps} field {f} rel err {err:.3e}")
+            assert err < tol
+        tol = TOLN[prec] if prec == 8 else 3e-5
+    steps = 2000
+    so, sg = o.state(), g.state()
+    assert sg["istep"] == so["istep"] == steps
+    assert sg["remap_count"] == so["remap_count"] and sg["halvings"] == so["halvings"]
+    sh_o, _ = o.shift()
+    sh_g, _ = g.shift()
+    assert np.array_equal(sh_o, sh_g)
+    dg, do = g.diagnostics(), o.diagnostics()
+    for k in ("energy", "enstrophy", "rho_var"):
+        print(f"  {k} {dg[k]:.12e} vs {do[k]:.12e}")
+        assert abs(dg[k] - do[k]) <= tol * abs(do[k])
+    g.close()
+    o.close()
+
+
+def test_large_grid_4096_fp32(gpu):
+    """The precision form at 4096^2 (FFT, bracket and update in FP32; shift,
+    time and the diagnostics' sums in FP64) for three steps from a seeded state
+    on 4 <= |k| <= 64 against the FP64 oracle, single steps and a captured
+    batch of two."""
+    n, steps = 4096, 3
+    print("4096^2 fp32 vs oracle", flush=True)
+    p = dict(BASE, sigma=1.5, nu=5e-5, kappa=5e-5, delt=5e-5)
+    o = oracle.Oracle(threads=16, nx=n, ny=n, **p)
+    ic = dict(seed=4096, kmin=4.0, kmax=64.0, amp=1e-4, expo=0.0)
+    o.set_spectral("rho", oracle.seeded_spectrum(n, n, 2 * np.pi, 2 * np.pi, **ic))
+    o.step(steps)
+    for gs in (0, 2):
+        g = shrt.Shrt(nx=n, ny=n, precision=4, graph_steps=gs, **p)
+        g.initialize_seeded(**ic)
+        g.time_advance(steps)
+        for f, ring in ((shrt.OMEGA, "aomg"), (shrt.RHO, "arho")):
+            err = rel_l2(g.spectral(f), o.buffer(ring))
+            print(f"  graph_steps={gs} field {f} rel err {err:.3e}")
+            assert err < TOLN[4]
+        assert g.state()["istep"] == steps
+        g.close()
+    o.close()
+
+
+@pytest.mark.parametrize("prec", [8, 4])
 def test_reference_ic_and_steps(gpu, prec):
     n = 64
     o, g = pair(n, prec, noise=1, Lx=np.pi, Ly=np.pi, sigma=1.0, kappa=8e-5, cfl_num=0.5)
