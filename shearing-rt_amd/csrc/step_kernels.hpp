@@ -407,6 +407,10 @@ k_p1_pair(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
     p1_outputs<T, NY>(a, row, base, base_n0, sn, kxr, 2 * grp, 2 * grp + 2, a.store_phi, sm, tid);
 }
 
+// max(m, |x|) in the precision of its operands (NaN x leaves m).
+__device__ __forceinline__ float maxabs(float m, float x) { return fmaxf(m, fabsf(x)); }
+__device__ __forceinline__ double maxabs(double m, double x) { return fmax(m, fabs(x)); }
+
 // ---------------------------------------------------------------------------
 // P2
 // ---------------------------------------------------------------------------
@@ -442,8 +446,10 @@ k_p2(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
 
     // CFL maxima (add_v0 + pl_max_search): max|vx + sigma*yy[y]|, max|vy|, one
     // partial pair per workgroup (P3 reduces the array; no atomics)
-    const double sy = __dmul_rn(a.ph.sigma, (double)y * g.dy);
-    double m0 = 0.0, m1 = 0.0;
+    // (in the working precision, as add_v0 forms vx + sigma*yy in cureal; the
+    // maxima are exact and converted to FP64 once per workgroup)
+    const T sy = (T)__dmul_rn(a.ph.sigma, (double)y * g.dy);
+    T m0 = 0, m1 = 0;
 
     // Inverse x-transforms of G field pairs FA + i FB for this team's column.
     // Four-pass form: (vx, dw/dx), (vy, dw/dy), (vx, dr/dx), (vy, dr/dy), so
@@ -511,16 +517,16 @@ k_p2(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
         if (a.ph.linear) {
 #pragma unroll
             for (int m = 0; m < P; ++m) {
-                m0 = fmax(m0, fabs(__dadd_rn((double)v[m].x, sy)));
-                m1 = fmax(m1, fabs((double)v[m].y));
+                m0 = maxabs(m0, v[m].x + sy);
+                m1 = maxabs(m1, v[m].y);
             }
         } else if (THREE) {
             // bw holds vx until B_omega exists, br holds vy until B_rho replaces it
             if (pass == 0) {
 #pragma unroll
                 for (int m = 0; m < P; ++m) {
-                    m0 = fmax(m0, fabs(__dadd_rn((double)v[m].x, sy)));
-                    m1 = fmax(m1, fabs((double)v[m].y));
+                    m0 = maxabs(m0, v[m].x + sy);
+                    m1 = maxabs(m1, v[m].y);
                     vxs[m] = v[m].x;
                     br[m] = v[m].y;
                 }
@@ -534,13 +540,13 @@ k_p2(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
         } else if (pass == 0) {
 #pragma unroll
             for (int m = 0; m < P; ++m) {
-                m0 = fmax(m0, fabs(__dadd_rn((double)v[m].x, sy)));
+                m0 = maxabs(m0, v[m].x + sy);
                 bw[m] = v[m].x * v[m].y;
             }
         } else if (pass == 1) {
 #pragma unroll
             for (int m = 0; m < P; ++m) {
-                m1 = fmax(m1, fabs((double)v[m].x));
+                m1 = maxabs(m1, v[m].x);
                 bw[m] += v[m].x * v[m].y;
             }
         } else if (pass == 2) {
@@ -552,15 +558,14 @@ k_p2(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
         }
     }
 
-    m0 = wave_max(m0);
-    m1 = wave_max(m1);
+    double c0 = wave_max((double)m0), c1 = wave_max((double)m1);
     {
         const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
-        if ((threadIdx.x & 63) == 0) { red[w] = m0; red[16 + w] = m1; }
+        if ((threadIdx.x & 63) == 0) { red[w] = c0; red[16 + w] = c1; }
         __syncthreads();
         if (threadIdx.x == 0) {
-            for (int i = 1; i < nw; ++i) { m0 = fmax(m0, red[i]); m1 = fmax(m1, red[16 + i]); }
-            reinterpret_cast<double2*>(a.cflp)[q] = double2{m0, m1};
+            for (int i = 1; i < nw; ++i) { c0 = fmax(c0, red[i]); c1 = fmax(c1, red[16 + i]); }
+            reinterpret_cast<double2*>(a.cflp)[q] = double2{c0, c1};
         }
     }
     if (a.ph.linear) return;
