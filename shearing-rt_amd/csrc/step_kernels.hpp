@@ -660,17 +660,47 @@ __device__ __forceinline__ cplx<T> ab_bdf_update(int ist, bool plain, double dt,
 }
 
 // FP32 contexts: the state update (AB/BDF2 combination and implicit
-// diffusion) is evaluated in FP64 from the FP32 operands and rounded once, so
-// each step adds one rounding of the state instead of about five; the
-// tendency, which enters multiplied by the step, stays FP32.
+// diffusion) is evaluated in FP64 from the FP32 operands and rounded once,
+// stochastically, so each step adds one unbiased rounding of the state instead
+// of about five round-to-nearest ones whose sub-ulp bias accumulates over
+// thousands of small steps; the tendency, which enters multiplied by the step,
+// stays FP32.
 constexpr bool kWideUpdate = true;
 
-// The same update for an FP32 state evaluated in FP64 and rounded once: the
-// divisions by 3 and by the diffusion denominator become one reciprocal and
-// products (their FP64 rounding is far below the final FP32 rounding).
+// 32-bit integer hash (the "lowbias32" finaliser of Chris Wellons' hash
+// prospector: xor-shift / multiply rounds).
+__device__ __forceinline__ uint32_t hash32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+// FP64 -> FP32 with stochastic rounding driven by a deterministic key: x
+// rounds away from zero with probability equal to its distance from the float
+// below it in magnitude over the gap (Connolly, Higham and Mary, "Stochastic
+// rounding and its probabilistic backward error analysis", 2021), done by
+// adding a uniform 29-bit integer to the 29 mantissa bits of the double that
+// a float drops and clearing them, after which the conversion is exact.  The
+// expected result is x, so the sub-ulp changes of a slowly evolving FP32
+// state accumulate instead of being rounded away every step.  Infinities and
+// NaNs pass unchanged; results in the float subnormal range (no state value
+// here comes near it) round to nearest after the truncation.
+__device__ __forceinline__ float round_stochastic(double x, uint32_t key) {
+    const unsigned long long noise = isfinite(x) ? (hash32(key) >> 3) : 0u;
+    const unsigned long long b = ((unsigned long long)__double_as_longlong(x) + noise) & ~0x1fffffffull;
+    return (float)__longlong_as_double((long long)b);
+}
+
+// The same update for an FP32 state evaluated in FP64 and rounded once, with
+// stochastic rounding keyed by (step, field, mode): the divisions by 3 and by
+// the diffusion denominator become one reciprocal and products (their FP64
+// rounding is far below the final FP32 rounding).
 __device__ __forceinline__ cplx<float> ab_bdf_update_wide(int ist, bool plain, double dt, double diff, double k2,
                                                          cplx<float> a1f, cplx<float> a2f, cplx<float> d0f,
-                                                         cplx<float> d1f, cplx<float> d2f) {
+                                                         cplx<float> d1f, cplx<float> d2f, uint32_t key) {
     const cplx<double> a1 = {a1f.x, a1f.y}, a2 = {a2f.x, a2f.y};
     const cplx<double> d0 = {d0f.x, d0f.y}, d1 = {d1f.x, d1f.y}, d2 = {d2f.x, d2f.y};
     constexpr double third = 1.0 / 3.0;
@@ -697,10 +727,13 @@ __device__ __forceinline__ cplx<float> ab_bdf_update_wide(int ist, bool plain, d
     }
     if (!plain) {
         const double cd = ist == 0 ? dt : (2.0 / 3.0) * dt;
-        const double inv = 1.0 / (1.0 + cd * diff * k2);
+        const double den = 1.0 + cd * diff * k2;
+        // 1/den: FP32 reciprocal refined by one Newton step (relative error ~1e-14)
+        double inv = (double)__builtin_amdgcn_rcpf((float)den);
+        inv = inv * (2.0 - den * inv);
         r = {r.x * inv, r.y * inv};
     }
-    return {(float)r.x, (float)r.y};
+    return {round_stochastic(r.x, key), round_stochastic(r.y, key * 0x2c1b3c6du + 0x297a2d39u)};
 }
 
 // Bookkeeping of step n done by workgroup 0 of P3: the shift advance and remap
@@ -810,6 +843,7 @@ k_p3(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
     const int nrow_tile = min(KT, g.nkxh - tile * KT);
     const StepCtl* c = a.ctl_in;
     const long long n = c->istep;
+    const uint32_t step_key = hash32((uint32_t)n * 2u + (uint32_t)f);   // stochastic-rounding stream of (step, field)
     cplx<T>* lds = sm + team * L;
     const TeamSync<S::TT> sync{};
 
@@ -898,7 +932,8 @@ k_p3(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
             }
             if constexpr (sizeof(T) == 4 && kWideUpdate) {
                 // FP32 state: the AB/BDF2 combination and the implicit diffusion in FP64, rounded once
-                an = ab_bdf_update_wide(ist, plain, dt, diff, k2, B.a1[u], B.a2[u], d0, B.d1[u], B.d2[u]);
+                const uint32_t key = step_key + (uint32_t)(r * g.nky + iky) * 0x9e3779b1u;
+                an = ab_bdf_update_wide(ist, plain, dt, diff, k2, B.a1[u], B.a2[u], d0, B.d1[u], B.d2[u], key);
             } else {
                 an = ab_bdf_update<T>(ist, plain, dt, diff, k2, B.a1[u], B.a2[u], d0, B.d1[u], B.d2[u]);
             }
