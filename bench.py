@@ -94,7 +94,7 @@ def pmc_summary(fetch_csv, write_csv, cal_fetch_csv, cal_write_csv, cal_bytes):
     w_cal = cal_bytes / (copy_kib(per_kernel(cal_write_csv)) * 1024.0)
     fetch, write = per_kernel(fetch_csv), per_kernel(write_csv)
     out = {"calibration": {"copy_bytes": cal_bytes, "fetch_factor": f_cal, "write_factor": w_cal}}
-    pat = re.compile(r"k_(p[123])(?:_row|_pair)?<(float|double), (\d+)>")
+    pat = re.compile(r"k_(p[123])(?:_row|_pair)?<(float|double), (\d+)[,>]")
     for k, fv in fetch.items():
         m = pat.search(k)
         if not m or k not in write:
