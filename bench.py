@@ -157,7 +157,7 @@ CLOSURE_DEFAULT = dict(n_fine=256, n_coarse=64, n_mid=128, delt=1e-3, interval=2
 
 def run_closure(shrt, args):
     """Learned closure versus plain solvers: data from fine runs, device
-    training with early stopping, then held-out seeds judged against the
+    training with early stopping, then held-out seeds measured against the
     filtered fine run (enstrophy spectrum and rho relative L2)."""
     cl = _load("closure_pipe", os.path.join(REPO, "shearing-rt_amd", "closure.py"))
     kw = dict(CLOSURE_DEFAULT)

@@ -1,7 +1,7 @@
 """Learned closure pipeline: training data from the solver's own fine runs, the
 spectral-convolution network trained on the device (shrt_closure_*), and the
 coarse solver corrected by it, with the plain coarse and a finer plain solver
-as the fallbacks it is judged against.
+as the fallbacks it is measured against.
 
 Data.  A fine run (n_fine^2, FP32) is filtered to the coarse grid by keeping
 the coarse grid's retained modes (the same kx, ky, and the same shift since
@@ -16,9 +16,9 @@ interval * delt * net(state) to omega and rho.  The correction is skipped
 (fallback to the plain coarse step) when the network output is not finite or
 larger than `guard` times the field it corrects.
 
-Judge.  The filtered fine run: rho relative L2 on the coarse modes and the
-relative L2 of the shell-summed enstrophy spectrum Z(k) = 1/2 sum |omega|^2
-(Hermitian-weighted) over the coarse shells.
+Error measure.  Against the filtered fine run: rho relative L2 on the coarse
+modes and the relative L2 of the shell-summed enstrophy spectrum
+Z(k) = 1/2 sum |omega|^2 (Hermitian-weighted) over the coarse shells.
 """
 import time
 
