@@ -824,7 +824,10 @@ k_p3(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
     constexpr int LP = stage_stride<NY>();
     constexpr int NKY = NY / 3 + 1;                 // == nky for the supported (power-of-two) lengths
     constexpr int J = (NKY + TT - 1) / TT;          // row points per thread
-    constexpr int JB = sizeof(T) == 4 ? 3 : 2;      // points whose loads are in flight together
+    // points whose loads are in flight together: three in the plain FP32 pair
+    // kernel, two where the batched argument block, the four-row tile or the
+    // fused tail need the registers
+    constexpr int JB = (sizeof(T) == 4 && !BATCH && KT == 1 && !FUSE) ? 3 : 2;
     static_assert(KT == 1 || KT == kTile, "P3 workgroups hold one row pair or one H tile");
     static_assert(!FUSE || KT == 1, "the fused next-step P1 works on one row pair");
     __shared__ cplx<T> sm[KT * L];
