@@ -125,26 +125,47 @@ def cpu_baseline(orc, n, params, ic, budget_s):
 
 def run_ensemble(shrt, args):
     """Aggregate throughput of independent instances stepped by one captured
-    graph per batch (shrt_ensemble_*)."""
+    graph per batch (shrt_ensemble_*).  SPEC is NxCOUNT,... or mixed:COUNT
+    (sizes drawn uniformly from 128, 256, 512).  Per instance: shear rate
+    U(0, 2), g U(0.5, 2), nu = kappa log-uniform in [1e-4, 1e-3], CFL number
+    0.3, rho seeded on 2 <= |k| <= n/16.  The autotuner picks the variants of
+    each grid size once (on that size's first instance) for all its members."""
     rng = np.random.default_rng(7)
+    if args.ensemble.startswith("mixed:"):
+        sizes = [int(x) for x in rng.choice([128, 256, 512], int(args.ensemble.split(":")[1]))]
+    else:
+        sizes = []
+        for part in args.ensemble.split(","):
+            n, cnt = (int(x) for x in part.split("x"))
+            sizes += [n] * cnt
     members, points = [], 0
-    for part in args.ensemble.split(","):
-        n, cnt = (int(x) for x in part.split("x"))
-        for _ in range(cnt):
-            sigma = float(rng.uniform(0.0, 2.0))
-            g = shrt.Shrt(nx=n, ny=n, precision=shrt.FP32, graph_steps=0, Lx=2 * np.pi, Ly=2 * np.pi,
-                          delt=2e-4, cfl_num=0.3, nu=1e-4, kappa=1e-4, sigma=sigma, g=float(rng.uniform(0.5, 2)),
-                          rho0=1.0, rho0_prime=1.0, rho_eps1=1e-2, noise=0, linear=0)
-            g.initialize_seeded(seed=int(rng.integers(1 << 62)), kmin=2.0, kmax=max(2.0, n / 16), amp=1e-4, expo=-1.5)
-            members.append(g)
-            points += n * n
+    for n in sizes:
+        nu = float(10 ** rng.uniform(-4, -3))
+        g = shrt.Shrt(nx=n, ny=n, precision=shrt.FP32, graph_steps=0, Lx=2 * np.pi, Ly=2 * np.pi,
+                      delt=2e-4, cfl_num=0.3, nu=nu, kappa=nu, sigma=float(rng.uniform(0.0, 2.0)),
+                      g=float(rng.uniform(0.5, 2)), rho0=1.0, rho0_prime=1.0, rho_eps1=1e-2, noise=0, linear=0)
+        g.initialize_seeded(seed=int(rng.integers(1 << 62)), kmin=2.0, kmax=max(2.0, n / 16), amp=1e-4, expo=-1.5)
+        members.append(g)
+        points += n * n
+    tuned = {}
+    for n in sorted(set(sizes)):
+        first = members[sizes.index(n)]
+        t = first.autotune(reps=10)
+        tuned[n] = dict(variants=[t["p1"], t["p2"], t["p3"]], fuse=t["fuse"])
+        for g, m in zip(members, sizes):
+            if m == n:
+                g.set_variants(t["p1"], t["p2"], t["p3"])
+                g.set_fusion(t["fuse"])
+    log(f"ensemble: {len(members)} instances, per-size choices {tuned}")
     ens = shrt.Ensemble(members, steps_per_graph=args.graph_steps)
     ens.step(args.graph_steps)   # warm-up and capture
     steps = max(args.graph_steps, args.steps // args.graph_steps * args.graph_steps)
     ms = ens.time_steps(steps)
     out = {"metric": "ensemble grid-pt*steps/s", "value": points * steps / (ms * 1e-3),
            "unit": "grid-pt*steps/s", "instances": len(members), "spec": args.ensemble, "steps": steps,
-           "ms_per_step": ms / steps, "dtype": "fp32", "graph_steps": args.graph_steps}
+           "ms_per_step": ms / steps, "dtype": "fp32", "graph_steps": args.graph_steps,
+           "sizes": {str(n): sizes.count(n) for n in sorted(set(sizes))},
+           "tuned": {str(n): v for n, v in tuned.items()}}
     ens.close()
     for g in members:
         g.close()
