@@ -192,7 +192,9 @@ def run_closure(shrt, args):
     x, y = p.generate(range(100, 100 + ntrain))
     val = p.generate(range(800, 800 + nval))
     t_data = time.perf_counter() - t0
-    log(f"closure: {len(x)} training samples in {t_data:.1f} s")
+    log(f"closure: {len(x)} training samples ({p.dropped} non-finite dropped, fine-run CFL halvings "
+        f"{p.max_halvings}) in {t_data:.1f} s; input std "
+        f"{np.round(x[:, :3].std(axis=(0, 2)), 4)}, target std {np.round(y.std(axis=(0, 2)), 4)}")
     t0 = time.perf_counter()
     losses = p.fit(x, y, iters=iters, batch=batch, lr=lr, val=val)
     t_train = time.perf_counter() - t0

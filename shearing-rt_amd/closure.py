@@ -148,6 +148,7 @@ class Pipeline:
         for seed in seeds:
             sigma = cfg.sigma_of(seed)
             snaps = self.fine_snapshots(seed, cfg.snaps + 1)
+            self.max_halvings = max(getattr(self, "max_halvings", 0), max(sn[4] for sn in snaps))
             c = self.ctx(cfg.n_coarse, sigma)
             for j in range(cfg.snaps):
                 om, rh, t, sh, _ = snaps[j]
@@ -161,7 +162,10 @@ class Pipeline:
                 ys.append(np.stack([c.ktox(d_om).ravel(), c.ktox(d_rh).ravel()]))
             c.close()
         self.scale_in = saved
-        return np.array(xs), np.array(ys)
+        x, y = np.array(xs), np.array(ys)
+        ok = np.isfinite(x).all(axis=(1, 2)) & np.isfinite(y).all(axis=(1, 2))
+        self.dropped = int((~ok).sum())   # samples from runs that blew up
+        return x[ok], y[ok]
 
     def normalise(self, x, y=None):
         xn = x.copy()
@@ -201,9 +205,10 @@ class Pipeline:
             done += k
             score = float(np.mean(self.r2(xv, yv)))
             self.val_curve.append((done, score))
-            if score > best:
+            if np.isfinite(score) and score > best:
                 best, best_p = score, self.net.params()
-        self.net.set_params(best_p)
+        if best_p is not None:
+            self.net.set_params(best_p)
         self.val_best = best
         return np.concatenate(losses)
 
