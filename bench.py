@@ -11,8 +11,9 @@ in HBM before the timed region starts.
 A step is one time_advance: three fused kernels, replayed from a captured HIP
 graph.  The timed region is `--steps` steps between HIP events on the
 library's stream, after `--warmup` untimed steps.  Per-kernel durations come
-from a second, kernel-by-kernel pass with HIP events between launches; the
-roofline object is for the kernel with the largest average duration.
+from a second, kernel-by-kernel pass in which every dispatch records HIP
+events at its own start and end; the roofline object is for the kernel with
+the largest average duration.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """

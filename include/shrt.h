@@ -177,9 +177,11 @@ shrt_status shrt_get_halvings(shrt_ctx* ctx, shrt_halving* out, int cap, int* co
  * `nsteps` steps exactly as shrt_step_async does (graph replays when
  * graph_steps > 0) between two HIP events on the context's stream and returns
  * the elapsed milliseconds.  shrt_kernel_times launches `nsteps` steps kernel
- * by kernel with HIP events between the launches and returns the average
- * duration in milliseconds of each of the three step kernels (ms[0..2] =
- * y-inverse/Poisson pass, x pass with brackets, y-forward/update pass). */
+ * by kernel, each dispatch recording HIP events at its own start and end
+ * (hipExtLaunchKernelGGL, so launch gaps are excluded), and returns the
+ * average duration in milliseconds of each of the three step kernels
+ * (ms[0..2] = y-inverse/Poisson pass, x pass with brackets, y-forward/update
+ * pass). */
 shrt_status shrt_time_steps(shrt_ctx* ctx, int nsteps, double* ms_total);
 shrt_status shrt_kernel_times(shrt_ctx* ctx, int nsteps, double* ms3);
 

@@ -9,6 +9,13 @@
 
 namespace shrt {
 
+// Events the step-kernel launchers hand to the dispatch when set (per host
+// thread; defined in shrt_api.hip).
+struct LaunchTiming {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
+extern thread_local LaunchTiming g_launch_timing;
+
 template <typename T>
 struct Launch {
     static bool supported(int n);

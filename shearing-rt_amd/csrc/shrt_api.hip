@@ -46,6 +46,7 @@ constexpr int kDiagBlocks = 512;
 // Error text for entry points defined in other translation units (the closure).
 namespace shrt {
 void set_last_error(const std::string& msg) { g_last_error = msg; }
+thread_local LaunchTiming g_launch_timing;
 }  // namespace shrt
 
 struct shrt_ctx {
@@ -782,11 +783,11 @@ shrt_status shrt_set_fusion(shrt_ctx* c, int on) {
 
 shrt_status shrt_kernel_times(shrt_ctx* c, int nsteps, double* ms3) {
     if (!c || !ms3 || nsteps <= 0) return fail(SHRT_ERR_INVALID, "bad arguments");
-    std::vector<hipEvent_t> ev((size_t)nsteps * 4);
+    std::vector<hipEvent_t> ev((size_t)nsteps * 6);
     for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
     hipError_t err = hipSuccess;
     for (int k = 0; k < nsteps && err == hipSuccess; ++k) {
-        hipEvent_t* e = &ev[(size_t)k * 4];
+        hipEvent_t* e = &ev[(size_t)k * 6];
         const int par = c->cur;
         auto launch = [&](int which) -> hipError_t {
             const int sp = k == nsteps - 1;
@@ -801,11 +802,12 @@ shrt_status shrt_kernel_times(shrt_ctx* c, int nsteps, double* ms3) {
                  : which == 1 ? Launch<double>::p2(a, nullptr, 1, c->var.p2, c->stream)
                               : Launch<double>::p3(a, nullptr, 1, c->var.p3, c->stream);
         };
+        // each dispatch records its own start and end (e[2w], e[2w+1])
         for (int w = 0; w < 3 && err == hipSuccess; ++w) {
-            err = hipEventRecord(e[w], c->stream);
-            if (err == hipSuccess) err = launch(w);
+            g_launch_timing = LaunchTiming{e[2 * w], e[2 * w + 1]};
+            err = launch(w);
+            g_launch_timing = LaunchTiming{};
         }
-        if (err == hipSuccess) err = hipEventRecord(e[3], c->stream);
         c->cur ^= 1;
     }
     HIP_TRY(err);
@@ -814,7 +816,7 @@ shrt_status shrt_kernel_times(shrt_ctx* c, int nsteps, double* ms3) {
     for (int k = 0; k < nsteps; ++k)
         for (int w = 0; w < 3; ++w) {
             float ms = 0.f;
-            HIP_TRY(hipEventElapsedTime(&ms, ev[(size_t)k * 4 + w], ev[(size_t)k * 4 + w + 1]));
+            HIP_TRY(hipEventElapsedTime(&ms, ev[(size_t)k * 6 + 2 * w], ev[(size_t)k * 6 + 2 * w + 1]));
             acc[w] += ms;
         }
     for (auto& e : ev) (void)hipEventDestroy(e);

@@ -1,5 +1,7 @@
 // shrt_kern.hip -- kernel instances and launchers for one precision.
 // Built twice: -DSHRT_REAL=float and -DSHRT_REAL=double.
+#include <hip/hip_ext.h>
+
 #include "launch.hpp"
 
 #ifndef SHRT_REAL
@@ -46,6 +48,17 @@ bool dispatch(int n, F&& f) {
 
 hipError_t unsupported() { return hipErrorInvalidValue; }
 
+// A step-kernel launch; with timing events set (shrt_kernel_times) the
+// dispatch itself records them at its start and end (hipExtLaunchKernelGGL),
+// so the measured duration excludes the launch gaps around it.
+template <typename F, typename... Args>
+void step_launch(F kernel, const dim3& grid, const dim3& block, uint32_t shmem, hipStream_t s, Args... args) {
+    if (g_launch_timing.start != nullptr)
+        hipExtLaunchKernelGGL(kernel, grid, block, shmem, s, g_launch_timing.start, g_launch_timing.stop, 0, args...);
+    else
+        hipLaunchKernelGGL(kernel, grid, block, shmem, s, args...);
+}
+
 }  // namespace
 
 template <>
@@ -73,17 +86,17 @@ hipError_t Launch<R>::p1(const StepArgs<R>& a, const StepArgs<R>* dargs, int cou
         constexpr int N = decltype(nc)::value;
         if (variant == kP1Pair) {
             const dim3 grid(a.g.nkxh * (a.ph.linear ? 1 : 3), dargs ? count : 1);
-            if (dargs) hipLaunchKernelGGL((k_p1_pair<R, N, true>), grid, dim3(TeamShape<N>::TT), 0, s, a, dargs);
-            else hipLaunchKernelGGL((k_p1_pair<R, N, false>), grid, dim3(TeamShape<N>::TT), 0, s, a, dargs);
+            if (dargs) step_launch((k_p1_pair<R, N, true>), grid, dim3(TeamShape<N>::TT), 0, s, a, dargs);
+            else step_launch((k_p1_pair<R, N, false>), grid, dim3(TeamShape<N>::TT), 0, s, a, dargs);
         } else if (variant == kP1Row) {
             const dim3 grid((a.g.nkxh - 1) * (a.ph.linear ? 2 : 6) + (a.ph.linear ? 1 : 3), dargs ? count : 1);
-            if (dargs) hipLaunchKernelGGL((k_p1_row<R, N, true>), grid, dim3(TeamShape<N>::TT), 0, s, a, dargs);
-            else hipLaunchKernelGGL((k_p1_row<R, N, false>), grid, dim3(TeamShape<N>::TT), 0, s, a, dargs);
+            if (dargs) step_launch((k_p1_row<R, N, true>), grid, dim3(TeamShape<N>::TT), 0, s, a, dargs);
+            else step_launch((k_p1_row<R, N, false>), grid, dim3(TeamShape<N>::TT), 0, s, a, dargs);
         } else {
             const P1Grid pg(a.g.nkxh, a.ph.linear);
             const dim3 grid(pg.grid, dargs ? count : 1), block(kTile * TeamShape<N>::TT);
-            if (dargs) hipLaunchKernelGGL((k_p1<R, N, true>), grid, block, 0, s, a, dargs);
-            else hipLaunchKernelGGL((k_p1<R, N, false>), grid, block, 0, s, a, dargs);
+            if (dargs) step_launch((k_p1<R, N, true>), grid, block, 0, s, a, dargs);
+            else step_launch((k_p1<R, N, false>), grid, block, 0, s, a, dargs);
         }
     });
     return ok ? hipGetLastError() : unsupported();
@@ -96,11 +109,11 @@ hipError_t Launch<R>::p2(const StepArgs<R>& a, const StepArgs<R>* dargs, int cou
         constexpr int KT = p2_teams<R, N>();
         const dim3 grid(a.g.ny / KT, dargs ? count : 1), block(KT * Team<N, p2_points<R, N>()>::TT);
         if (variant == kP2Three) {
-            if (dargs) hipLaunchKernelGGL((k_p2<R, N, true, true>), grid, block, 0, s, a, dargs);
-            else hipLaunchKernelGGL((k_p2<R, N, false, true>), grid, block, 0, s, a, dargs);
+            if (dargs) step_launch((k_p2<R, N, true, true>), grid, block, 0, s, a, dargs);
+            else step_launch((k_p2<R, N, false, true>), grid, block, 0, s, a, dargs);
         } else {
-            if (dargs) hipLaunchKernelGGL((k_p2<R, N, true, false>), grid, block, 0, s, a, dargs);
-            else hipLaunchKernelGGL((k_p2<R, N, false, false>), grid, block, 0, s, a, dargs);
+            if (dargs) step_launch((k_p2<R, N, true, false>), grid, block, 0, s, a, dargs);
+            else step_launch((k_p2<R, N, false, false>), grid, block, 0, s, a, dargs);
         }
     });
     return ok ? hipGetLastError() : unsupported();
@@ -113,8 +126,8 @@ hipError_t Launch<R>::p3(const StepArgs<R>& a, const StepArgs<R>* dargs, int cou
         auto go = [&](auto ktc) {
             constexpr int KT = decltype(ktc)::value;
             const dim3 grid(2 * ((a.g.nkxh + KT - 1) / KT), dargs ? count : 1), block(KT * TeamShape<N>::TT);
-            if (dargs) hipLaunchKernelGGL((k_p3<R, N, true, KT, false>), grid, block, 0, s, a, dargs);
-            else hipLaunchKernelGGL((k_p3<R, N, false, KT, false>), grid, block, 0, s, a, dargs);
+            if (dargs) step_launch((k_p3<R, N, true, KT, false>), grid, block, 0, s, a, dargs);
+            else step_launch((k_p3<R, N, false, KT, false>), grid, block, 0, s, a, dargs);
         };
         if (variant == kP3Pair) go(std::integral_constant<int, 1>{});
         else go(std::integral_constant<int, kTile>{});
@@ -127,8 +140,8 @@ hipError_t Launch<R>::p31(const StepArgs<R>& a, const StepArgs<R>* dargs, int co
     bool ok = dispatch(a.g.ny, [&](auto nc) {
         constexpr int N = decltype(nc)::value;
         const dim3 grid(2 * a.g.nkxh, dargs ? count : 1), block(TeamShape<N>::TT);
-        if (dargs) hipLaunchKernelGGL((k_p3<R, N, true, 1, true>), grid, block, 0, s, a, dargs);
-        else hipLaunchKernelGGL((k_p3<R, N, false, 1, true>), grid, block, 0, s, a, dargs);
+        if (dargs) step_launch((k_p3<R, N, true, 1, true>), grid, block, 0, s, a, dargs);
+        else step_launch((k_p3<R, N, false, 1, true>), grid, block, 0, s, a, dargs);
     });
     return ok ? hipGetLastError() : unsupported();
 }
