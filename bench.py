@@ -215,6 +215,33 @@ def run_closure(shrt, args):
     print(json.dumps(out), flush=True)
 
 
+def run_vendor_fft(shrt, args):
+    """The reference's per-step transforms (six 2-D C2R, two R2C) by the vendor
+    FFT library (shearing-rt_amd/vendor_fft_bench), beside this library's whole
+    step at the same size and precision."""
+    import subprocess
+    exe = os.path.join(REPO, "shearing-rt_amd", "vendor_fft_bench")
+    if not os.path.exists(exe):
+        raise RuntimeError(f"{exe} is missing: build it with `make -C shearing-rt_amd`")
+    n = args.n
+    p = subprocess.run([exe, str(n), str(n), "100", args.precision], capture_output=True, text=True, timeout=300)
+    if p.returncode != 0:
+        raise RuntimeError(p.stderr)
+    vendor = json.loads(p.stdout.strip().splitlines()[-1])
+    prec = shrt.FP32 if args.precision == "fp32" else shrt.FP64
+    g = shrt.Shrt(nx=n, ny=n, precision=prec, graph_steps=args.graph_steps, Lx=2 * np.pi, Ly=2 * np.pi, delt=1e-4,
+                  cfl_num=0.35, nu=1e-4, kappa=1e-4, sigma=1.0, g=1.0, rho0=1.0, rho0_prime=1.0, rho_eps1=1e-2,
+                  noise=0, linear=0)
+    g.initialize_seeded(seed=20250101, kmin=2.0, kmax=32.0, amp=1e-4, expo=-1.5)
+    g.time_advance(args.graph_steps * 5)
+    steps = max(args.graph_steps, 1000 // args.graph_steps * args.graph_steps)
+    ms = g.time_steps(steps) / steps
+    g.close()
+    print(json.dumps({"metric": "vendor FFT comparison", "n": n, "precision": args.precision, "vendor": vendor,
+                      "shrt_ms_per_step": ms,
+                      "vendor_step_ffts_over_shrt_step": vendor["ms_reference_step_ffts"] / ms}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -239,6 +266,8 @@ def main():
     ap.add_argument("--autotune-report", default="", metavar="SIZES",
                     help="only run shrt_autotune for each square size (e.g. 256,512,1024,2048,4096) and "
                          "print the per-variant kernel times as JSON lines, then exit")
+    ap.add_argument("--vendor-fft", action="store_true",
+                    help="only time the reference's per-step FFTs with the vendor library (comparison) and exit")
     ap.add_argument("--closure", default="", metavar="JSON",
                     help="only run the learned-closure experiment (JSON overrides of its configuration, e.g. '{}') "
                          "and exit")
@@ -280,6 +309,9 @@ def main():
         return
     if args.closure:
         run_closure(shrt, args)
+        return
+    if args.vendor_fft:
+        run_vendor_fft(shrt, args)
         return
     if args.autotune_report:
         prec = shrt.FP32 if args.precision == "fp32" else shrt.FP64
