@@ -45,9 +45,15 @@ typedef enum { SHRT_FP32 = 4, SHRT_FP64 = 8 } shrt_precision;
 /* Spectral fields (the reference's dv_aomg0 / dv_aphi / dv_arho0). */
 typedef enum { SHRT_OMEGA = 0, SHRT_PHI = 1, SHRT_RHO = 2 } shrt_field;
 
-/* Physical fields: omega, phi, rho (output_fields, file_access.cu:250-263) and the
- * velocities vx, vy of the last CFL check (time_integral.cu:354). */
-typedef enum { SHRT_P_OMEGA = 0, SHRT_P_PHI = 1, SHRT_P_RHO = 2, SHRT_P_VX = 3, SHRT_P_VY = 4 } shrt_pfield;
+/* Physical fields: omega, phi, rho (output_fields, file_access.cu:250-263), the
+ * velocities vx, vy of the last CFL check (time_integral.cu:354), and the
+ * advection term vx*d(omega)/dx + vy*d(omega)/dy of the current state that the
+ * next step computes and output_pbk writes (poisson_bracket_shear,
+ * shear.cu:395-434, called at time_integral.cu:500-505; the velocities are those
+ * of phi = omega/kperp2 with the current shift, as check_cfl makes them). */
+typedef enum {
+    SHRT_P_OMEGA = 0, SHRT_P_PHI = 1, SHRT_P_RHO = 2, SHRT_P_VX = 3, SHRT_P_VY = 4, SHRT_P_BRACKET = 5
+} shrt_pfield;
 
 /* Ring buffers of the AB3/BDF2 history (fields.cu:21-25). */
 typedef enum { SHRT_RING_AOMG = 0, SHRT_RING_DOMG = 1, SHRT_RING_ARHO = 2, SHRT_RING_DRHO = 3 } shrt_ring;
@@ -140,8 +146,8 @@ shrt_status shrt_set_buffer(shrt_ctx* ctx, int ring, int slot, const void* host)
  * reals.  lab_frame = 0: ktox (fft.cu:170-204) in the sheared frame; lab_frame
  * = 1: what output_fields writes (file_access.cu:250-259), ktox_shear
  * (shear.cu:467-478) with the current ky_shift when sigma != 0, ktox otherwise.
- * The velocities (those of the last CFL check, time_integral.cu:354) exist in
- * the sheared frame only. */
+ * The velocities (those of the last CFL check, time_integral.cu:354) and the
+ * bracket exist in the sheared frame only. */
 shrt_status shrt_get_physical(shrt_ctx* ctx, int pfield, int lab_frame, void* host);
 
 /* xtok / ktox (fft.h:17-25) on host buffers. */

@@ -119,6 +119,7 @@ struct Driver {
     const Grid& g;
     std::vector<R> ao, ap, ar;             // spectral, interleaved complex
     std::vector<R> omg, phi, rho, vx, vy;  // physical
+    std::vector<R> pbk;                    // advection term of the step about to run
     int printed_halvings = 0;
     bool energy_started = false;
 
@@ -126,6 +127,7 @@ struct Driver {
         const size_t M = (size_t)g.nkx * g.nky, N = (size_t)g.nx * g.ny;
         ao.resize(2 * M); ap.resize(2 * M); ar.resize(2 * M);
         omg.resize(N); phi.resize(N); rho.resize(N); vx.resize(N); vy.resize(N);
+        pbk.resize(N);
     }
 
     shrt_time now() {
@@ -218,11 +220,23 @@ struct Driver {
                 const double room = (horizon - time) / t.delt - 2.0;
                 if (room > 1.0) k = (long long)std::fmin(room, 1e6);
             }
+            // advect_omg (time_integral.cu:500-505) writes the bracket it computes
+            // when the step ends past the next output time; check_cfl can only
+            // shorten delt, so a step that may end there takes the bracket of the
+            // current state first (k is 1 there: the horizon keeps two steps).
+            const bool pbk_now = c.write_fields && !c.linear_flag;
+            const bool pbk_due = pbk_now && time != 0 && time + t.delt > next_output_time;
+            if (pbk_now && time == 0) output_pbk<R>(g, 0.0, pbk.data());
+            if (pbk_due) {
+                k = 1;
+                check(shrt_get_physical(ctx, SHRT_P_BRACKET, 0, pbk.data()), "shrt_get_physical");
+            }
             check(shrt_step(ctx, (int)k), "shrt_step");
             t = now();
             istep = (int)t.istep;
             time = t.time;
             print_halvings();
+            if (pbk_due && time > next_output_time) output_pbk<R>(g, time, pbk.data());
 
             if (istep == 1) k_data_bef_now(time, istep);
             if (c.write_fields && time > next_output_time) {

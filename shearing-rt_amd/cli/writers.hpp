@@ -7,6 +7,7 @@
 //   en_spectral     kx_ensp*.dat / ky_ensp*.dat           (:350-447)
 //   ks_reim         ksre_t*.dat / ksim_t*.dat             (:451-488)
 //   k_data_bef/aft  befk_n*_t*.dat / aftk_n*_t*.dat       (:492-626)
+//   output_pbk      pbk.txt / pbk_t%09.6f.dat             (:630-667)
 //   energy series   energy.dat (new): time E Z P, FP64 compensated sums
 //
 // The element type R is the run's precision (float or double); every value
@@ -185,6 +186,20 @@ void k_data_aft(const Grid& g, double time, int istep, int nwrite, const R* ao, 
         return;
     }
     k_data_dump(g, fmt_name("aftk_n%05d_t%09.6f.dat", istep / nwrite, time), ao, ap, ar);
+}
+
+// output_pbk (file_access.cu:630-667): the empty marker at time 0, else the
+// advection term in the periodic (nx+1) x (ny+1) table, one value per row.
+template <typename R>
+void output_pbk(const Grid& g, double time, const R* pb) {
+    if (time == 0) {
+        std::fclose(open_or_exit("pbk.txt", "w+"));
+        return;
+    }
+    FILE* fp = open_or_exit(fmt_name_t("pbk_t%09.6f.dat", time), "w+");
+    const R* const f1[1] = {pb};
+    write_table<R, 1>(fp, g, f1);
+    std::fclose(fp);
 }
 
 inline void energy_series(bool first, double time, double E, double Z, double P) {

@@ -324,6 +324,18 @@ __global__ void k_fill_zero(cplx<T>* p, long long n) {
     if (t < n) p[t] = czero<T>();
 }
 
+// out = (accumulate ? out : 0) + (negate ? -a*b : a*b), pointwise on physical
+// fields: the product kernel of poisson_bracket_shear (shear.cu:415-426), one
+// product per launch.
+template <typename T>
+__global__ void k_mul_acc(const T* __restrict__ a, const T* __restrict__ b, T* __restrict__ out, long long n,
+                          int negate, int accumulate) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const T p = negate ? -a[t] : a[t];
+    out[t] = accumulate ? fma(p, b[t], out[t]) : p * b[t];
+}
+
 // Neumaier-compensated FP64 accumulation.
 struct KSum {
     double s, c;

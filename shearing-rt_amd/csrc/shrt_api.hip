@@ -302,6 +302,11 @@ hipError_t ktox_shear_dev(shrt_ctx* c, const void* spec, void* phys, const doubl
     return c->prec == 4 ? ktox_shear_dev<float>(c, spec, phys, s, jmp) : ktox_shear_dev<double>(c, spec, phys, s, jmp);
 }
 
+hipError_t mul_acc(shrt_ctx* c, const void* a, const void* b, void* out, long long n, int negate, int accumulate) {
+    return c->prec == 4 ? Launch<float>::mul_acc(a, b, out, n, negate, accumulate, c->stream)
+                        : Launch<double>::mul_acc(a, b, out, n, negate, accumulate, c->stream);
+}
+
 hipError_t zero_buffer(shrt_ctx* c, void* p, long long nelem) {
     return c->prec == 4 ? Launch<float>::fill_zero(p, nelem, c->stream)
                         : Launch<double>::fill_zero(p, nelem, c->stream);
@@ -957,6 +962,40 @@ shrt_status shrt_set_buffer(shrt_ctx* c, int ring, int j, const void* host) {
     return write_ctl(c, v);
 }
 
+// The advection term of the current state, vx*d(omega)/dx + vy*d(omega)/dy in
+// the sheared frame, as the next step's advect_omg computes it
+// (poisson_bracket_shear, shear.cu:395-434): phi = omega/kperp2 with the
+// current shift (neg_lapinv_shear, shear.cu:367-374), vx = ktox(i*kys*phi),
+// vy = ktox(-i*kx*phi) (get_vector_shear, shear.cu:378-391), and the
+// derivatives of omega.  ktox of -i*kx*omega is -d(omega)/dx, hence the negated
+// first product.  The two physical factors live in G beyond the nkxh*ny
+// elements that ktox_dev uses for its intermediate.
+static shrt_status bracket_to_host(shrt_ctx* c, const StepCtl& v, void* host, int lab_frame) {
+    if (lab_frame) return fail(SHRT_ERR_INVALID, "the bracket is output in the sheared frame only");
+    if (c->ph.linear) return fail(SHRT_ERR_INVALID, "a linear run computes no bracket (time_integral.cu:496-498)");
+    const size_t pb = phys_bytes(c), off = (size_t)c->g.I * c->esz;
+    if (off + 2 * pb > 6 * (size_t)c->g.I * c->esz) return fail(SHRT_ERR_UNSUPPORTED, "bracket scratch too small for this grid");
+    const int slot = slot_a1(v.istep);
+    const int* jmp;
+    shrt_status st = view_for(c, v, v.slot_epoch_a[slot], &jmp);
+    if (st != SHRT_OK) return st;
+    const double* s_now = c->s + (size_t)c->cur * c->g.nkx;
+    void* f0 = static_cast<char*>(c->G) + off;
+    void* f1 = static_cast<char*>(f0) + pb;
+    const long long n = (long long)c->g.nx * c->g.ny;
+    HIP_TRY(view_copy(c, c->ring[0][slot], c->tmp_spec, jmp));   // omega in the current frame
+    HIP_TRY(phi_from_omega(c, c->tmp_spec, c->tmp_spec2, s_now));
+    HIP_TRY(ktox_dev(c, c->tmp_spec2, f0, s_now, nullptr, 1));   // vx
+    HIP_TRY(ktox_dev(c, c->tmp_spec, f1, s_now, nullptr, 2));    // -d(omega)/dx
+    HIP_TRY(mul_acc(c, f0, f1, c->tmp_phys, n, 1, 0));
+    HIP_TRY(ktox_dev(c, c->tmp_spec2, f0, s_now, nullptr, 2));   // vy
+    HIP_TRY(ktox_dev(c, c->tmp_spec, f1, s_now, nullptr, 1));    // d(omega)/dy
+    HIP_TRY(mul_acc(c, f0, f1, c->tmp_phys, n, 0, 1));
+    HIP_TRY(hipMemcpyAsync(host, c->tmp_phys, pb, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return SHRT_OK;
+}
+
 shrt_status shrt_get_physical(shrt_ctx* c, int pf, int lab_frame, void* host) {
     if (!c || !host) return fail(SHRT_ERR_INVALID, "null argument");
     if (lab_frame && (pf == SHRT_P_VX || pf == SHRT_P_VY))
@@ -976,6 +1015,7 @@ shrt_status shrt_get_physical(shrt_ctx* c, int pf, int lab_frame, void* host) {
         case SHRT_P_PHI: src = c->aphi; ep = v.phi_epoch; break;
         case SHRT_P_VX: src = c->aphi; mode = 1; s = s_phi; break;   // raw aphi of the last CFL check
         case SHRT_P_VY: src = c->aphi; mode = 2; s = s_phi; break;
+        case SHRT_P_BRACKET: return bracket_to_host(c, v, host, lab_frame);
         default: return fail(SHRT_ERR_INVALID, "unknown physical field");
     }
     const int* jmp = nullptr;

@@ -227,6 +227,16 @@ hipError_t Launch<R>::fill_zero(void* p, long long n, hipStream_t s) {
 }
 
 template <>
+hipError_t Launch<R>::mul_acc(const void* a, const void* b, void* out, long long n, int negate, int accumulate,
+                              hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_mul_acc<R>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                       static_cast<const R*>(a), static_cast<const R*>(b), static_cast<R*>(out), n, negate,
+                       accumulate);
+    return hipGetLastError();
+}
+
+template <>
 hipError_t Launch<R>::copy16(const void* src, void* dst, long long nwords, hipStream_t s) {
     hipLaunchKernelGGL((k_copy16<R>), dim3(4096), dim3(256), 0, s, static_cast<const uint4*>(src),
                        static_cast<uint4*>(dst), nwords);

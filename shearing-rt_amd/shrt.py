@@ -20,7 +20,7 @@ HEADER = os.path.join(REPO, "include", "shrt.h")
 
 FP32, FP64 = 4, 8
 OMEGA, PHI, RHO = 0, 1, 2
-P_OMEGA, P_PHI, P_RHO, P_VX, P_VY = 0, 1, 2, 3, 4
+P_OMEGA, P_PHI, P_RHO, P_VX, P_VY, P_BRACKET = 0, 1, 2, 3, 4, 5
 RING = {"aomg": 0, "domg": 1, "arho": 2, "drho": 3}
 
 _lib = None

@@ -83,7 +83,9 @@ def test_output_schedule(run_dir):
         assert f"vel_t{t}.dat" in names and f"ksre_t{t}.dat" in names and f"ksim_t{t}.dat" in names
     assert {"n00000_t00.000000.dat", "n00008_t00.008000.dat", "n00015_t00.015000.dat"} <= names
     assert len([n for n in names if n.startswith("n000")]) == 3
-    assert {"kx_ensp.txt", "ky_ensp.txt", "after_kdata.txt", "max_amp.dat", "energy.dat"} <= names
+    assert {"kx_ensp.txt", "ky_ensp.txt", "after_kdata.txt", "pbk.txt", "max_amp.dat", "energy.dat"} <= names
+    # output_pbk at the output steps only, labelled with the time after the step
+    assert sorted(n for n in names if n.startswith("pbk_t")) == [f"pbk_t{t}.dat" for _, t in OUT]
     assert {"kx_ensp_t00.008000.dat", "ky_ensp_t00.015000.dat"} <= names
     # k_data_bef at istep 0 and istep 1 only (sigma != 0, nonlinear)
     assert sorted(n for n in names if n.startswith("befk")) == ["befk_n00000_t00.000000.dat",
@@ -134,3 +136,25 @@ def test_spectral_dumps(run_dir):
     assert ks.shape[0] == nkx * nky
     assert np.all(np.diff(ks[::nky, 0]) > 0)   # rows in ascending kx (kx_index order)
     assert rel_l2(np.sort(ks[:, 2]), np.sort(np.abs(o.buffer("aomg").real).ravel())) < 1e-9
+
+
+def test_pbk_files_match_oracle(run_dir):
+    """pbk_t*.dat holds the bracket advect_omg computed in the step that ended
+    past the output time: that of the state before the step."""
+    d, _ = run_dir
+    o = oracle.Oracle(nx=32, ny=32, Lx=np.pi, Ly=np.pi, delt=1e-3, cfl_num=0.5, nu=1e-3, kappa=8e-5,
+                      sigma=1.0, g=1.0, rho0=1.0, rho0_prime=1.0, rho_eps1=1e-2, noise=1, linear=0)
+    o.initialize_reference()
+    assert os.path.getsize(d / "pbk.txt") == 0
+    done = 0
+    for istep, t in OUT:
+        o.step(istep - 1 - done)
+        done = istep - 1
+        o.check_cfl()
+        ref = o.poisson_bracket(o.buffer("aomg"))
+        a = read_table(d / f"pbk_t{t}.dat", 3)
+        grid = a.reshape(33, 33, 3)
+        assert np.array_equal(grid[32, :32, 2], grid[0, :32, 2])
+        assert np.array_equal(grid[:32, 32, 2], grid[:32, 0, 2])
+        assert grid[32, 32, 2] == grid[0, 0, 2]
+        assert rel_l2(grid[:32, :32, 2], ref) < 1e-9, t

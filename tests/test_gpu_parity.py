@@ -428,6 +428,36 @@ def test_lab_frame_output(gpu, prec):
 
 
 @pytest.mark.parametrize("prec", [8, 4])
+@pytest.mark.parametrize("nx,ny", [(32, 32), (64, 128)])
+def test_bracket_output(gpu, prec, nx, ny):
+    """The advection term that output_pbk writes: the bracket of the current
+    state as the next step computes it (velocities of phi = omega/kperp2 with
+    the current shift), before and right after a remap, against the oracle's
+    check_cfl velocities and poisson_bracket."""
+    print(f"bracket output nx={nx} ny={ny} prec={prec}", flush=True)
+    o, g = pair(nx, prec, ny=ny, sigma=1.0, delt=0.01, cfl_num=5.0, nu=1e-2, kappa=1e-2)
+    ic = dict(seed=19, kmin=1.0, kmax=6.0, amp=1e-1, expo=-1.0)
+    o.set_spectral("rho", oracle.seeded_spectrum(nx, ny, 2 * np.pi, 2 * np.pi, **ic))
+    g.initialize_seeded(**ic)
+    for steps in (5, 32, 63):   # the last run ends on the remap step (index 99)
+        o.step(steps)
+        g.time_advance(steps)
+        o.check_cfl()           # only sets aphi, vx, vy from the current omega
+        ref = o.poisson_bracket(o.buffer("aomg"))
+        got = g.physical(shrt.P_BRACKET)
+        err = rel_l2(got, ref)
+        assert err < TOLN[prec], (steps, err)
+    assert g.state()["remap_count"] == 1
+    with pytest.raises(shrt.ShrtError):
+        g.physical(shrt.P_BRACKET, lab_frame=1)
+    g.close()
+    lin = shrt.Shrt(nx=32, ny=32, precision=prec, graph_steps=0, **dict(BASE, linear=1))
+    with pytest.raises(shrt.ShrtError):
+        lin.physical(shrt.P_BRACKET)
+    lin.close()
+
+
+@pytest.mark.parametrize("prec", [8, 4])
 def test_kernel_variants(gpu, prec):
     """Every combination of the pass variants matches the oracle and the others
     (to rounding: the variants are compiled in different kernels, so the
