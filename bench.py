@@ -124,6 +124,45 @@ def cpu_baseline(orc, n, params, ic, budget_s):
                       f"{threads} OpenMP threads, {dt:.2f} s"}
 
 
+# Physics and initial condition of the timed workload.
+PARAMS = dict(Lx=2 * np.pi, Ly=2 * np.pi, delt=1e-4, cfl_num=0.35, nu=1e-4, kappa=1e-4,
+              sigma=1.0, g=1.0, rho0=1.0, rho0_prime=1.0, rho_eps1=1e-2, noise=0, linear=0)
+IC = dict(seed=20250101, kmin=2.0, kmax=32.0, amp=1e-4, expo=-1.5)
+
+
+def cpu_scan(orc, params, ic, budget_s):
+    """The FP64 oracle on the host cores at one thread and at all of them (at
+    most 16): 128^2 with the small-grid physics for 2000 steps in full, 1024^2
+    and 2048^2 with the bench physics for as many steps as fit in `budget_s`
+    seconds.  One JSON line per (size, threads)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), model)
+    except OSError:
+        pass
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    top = max(1, min(16, avail))
+    small = dict(params, delt=1e-3, nu=1e-3, kappa=1e-3, sigma=0.5)
+    small_ic = dict(seed=ic["seed"], kmin=2.0, kmax=8.0, amp=1e-3, expo=-1.0)
+    for n, full, p, q in ((128, 2000, small, small_ic), (1024, 0, params, ic), (2048, 0, params, ic)):
+        for threads in sorted({1, top}):
+            o = orc.Oracle(threads=threads, nx=n, ny=n, **p)
+            o.set_spectral("rho", orc.seeded_spectrum(n, n, p["Lx"], p["Ly"], **q))
+            t0 = time.perf_counter()
+            o.step(1)
+            one = time.perf_counter() - t0
+            steps = full or int(max(2, min(1000, budget_s / max(one, 1e-6))))
+            t0 = time.perf_counter()
+            o.step(steps)
+            dt = time.perf_counter() - t0
+            o.close()
+            print(json.dumps({"metric": "FP64 oracle throughput", "n": n, "threads": threads, "steps": steps,
+                              "seconds": dt, "ms_per_step": dt * 1e3 / steps,
+                              "grid_pt_steps_per_s": n * n * steps / dt, "cpu": model,
+                              "cpus_available": avail}), flush=True)
+
+
 def run_ensemble(shrt, args):
     """Aggregate throughput of independent instances stepped by one captured
     graph per batch (shrt_ensemble_*).  SPEC is NxCOUNT,... or mixed:COUNT
@@ -271,6 +310,9 @@ def main():
     ap.add_argument("--closure", default="", metavar="JSON",
                     help="only run the learned-closure experiment (JSON overrides of its configuration, e.g. '{}') "
                          "and exit")
+    ap.add_argument("--cpu-scan", action="store_true",
+                    help="only time the FP64 oracle on the host cores (1 and all threads; 128^2 x 2000 steps, "
+                         "1024^2 and 2048^2 on --cpu-budget seconds each) and exit")
     ap.add_argument("--ensemble", default="", metavar="SPEC",
                     help="only time an ensemble, e.g. 128x24,256x24,512x16 (grid x count; FP32, mixed "
                          "shear rates and seeds, one captured graph per --graph-steps steps) and exit")
@@ -287,8 +329,11 @@ def main():
               "the single-instance step is not sharded across GPUs", file=sys.stderr)
         sys.exit(2)
 
-    shrt = _load("shrt_binding", os.path.join(REPO, "shearing-rt_amd", "shrt.py"))
     orc = _load("shrt_oracle_py", os.path.join(REPO, "oracle", "oracle.py"))
+    if args.cpu_scan:
+        cpu_scan(orc, PARAMS, IC, args.cpu_budget)
+        return
+    shrt = _load("shrt_binding", os.path.join(REPO, "shearing-rt_amd", "shrt.py"))
     if shrt.lib().shrt_device_count() < 1:
         raise RuntimeError("no HIP device visible")
 
@@ -331,9 +376,8 @@ def main():
 
     n = args.n
     prec = shrt.FP32 if args.precision == "fp32" else shrt.FP64
-    params = dict(Lx=2 * np.pi, Ly=2 * np.pi, delt=1e-4, cfl_num=0.35, nu=1e-4, kappa=1e-4,
-                  sigma=1.0, g=1.0, rho0=1.0, rho0_prime=1.0, rho_eps1=1e-2, noise=0, linear=0)
-    ic = dict(seed=20250101, kmin=2.0, kmax=32.0, amp=1e-4, expo=-1.5)
+    params = dict(PARAMS)
+    ic = dict(IC)
 
     log(f"bench: {n}^2 {args.precision}, graph_steps={args.graph_steps}, warmup={args.warmup}, steps={args.steps}")
     g = shrt.Shrt(nx=n, ny=n, precision=prec, graph_steps=args.graph_steps, **params)
