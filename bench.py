@@ -337,9 +337,10 @@ def main():
     if shrt.lib().shrt_device_count() < 1:
         raise RuntimeError("no HIP device visible")
 
-    def copy_gbs(nbytes, reps):
+    def copy_gbs(nbytes, reps, stream=False):
         ms = ctypes.c_double()
-        st = shrt.lib().shrt_copy_bench(nbytes, reps, ctypes.byref(ms))
+        fn = shrt.lib().shrt_stream_copy_bench if stream else shrt.lib().shrt_copy_bench
+        st = fn(nbytes, reps, ctypes.byref(ms))
         if st != 0:
             raise RuntimeError(shrt.lib().shrt_last_error().decode())
         return ms.value, 2 * nbytes / (ms.value * 1e-3) / 1e9
@@ -397,7 +398,9 @@ def main():
     log(f"bench: kernel ms per launch p1={kt[0]:.4f} p2={kt[1]:.4f} p3={kt[2]:.4f}")
     g.close()
     copy_ms, copy_bw = copy_gbs(1 << 30, 10)
-    log(f"bench: stream copy 1 GiB: {copy_ms:.3f} ms, {copy_bw:.1f} GB/s (read + write)")
+    scopy_ms, scopy_bw = copy_gbs(1 << 30, 10, stream=True)
+    log(f"bench: plain copy 1 GiB: {copy_ms:.3f} ms, {copy_bw:.1f} GB/s; streaming (nontemporal) copy: "
+        f"{scopy_ms:.3f} ms, {scopy_bw:.1f} GB/s (read + write)")
 
     c = 8 if prec == shrt.FP32 else 16
     s = shrt.sizes(n, n)
@@ -433,8 +436,10 @@ def main():
                      "bytes_per_launch": kb[dname], "ms_per_launch": float(kt[dom])},
         "kernels_ms": {k: float(v) for k, v in zip(names, kt)},
         "stream_copy_GBs": copy_bw,
+        "stream_copy_nt_GBs": scopy_bw,
         "step_roofline": {"bytes_per_step": step_bytes,
                           "frac": step_bytes / (ms_step * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                          "frac_of_stream_copy_nt": step_bytes / (ms_step * 1e-3) / 1e9 / scopy_bw,
                           "bytes_per_step_c14M10I": baseline_formula,
                           "frac_c14M10I": baseline_formula / (ms_step * 1e-3) / 1e9 / PEAK_HBM_GBS},
         "final_state": {"istep": st["istep"], "time": st["time"], "delt": st["delt"],

@@ -197,6 +197,11 @@ shrt_status shrt_kernel_times(shrt_ctx* ctx, int nsteps, double* ms3);
  * reference point that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings. */
 shrt_status shrt_copy_bench(long long bytes, int reps, double* ms);
 
+/* Benchmark support (no reference counterpart): the same measurement with the
+ * fastest copy kernel found on MI355X (four 16-byte words in flight per
+ * thread, nontemporal loads and stores), i.e. the achievable HBM copy rate. */
+shrt_status shrt_stream_copy_bench(long long bytes, int reps, double* ms);
+
 /* Autotuning (new; no reference counterpart).  Each pass of the step has
  * kernel variants: P1 one (row pair, output) per workgroup (0), a tile of four
  * row pairs (1) or one (row pair, field group) with its two outputs (2); P2

@@ -44,6 +44,7 @@ struct Launch {
     static hipError_t mul_acc(const void* a, const void* b, void* out, long long n, int negate, int accumulate,
                               hipStream_t s);
     static hipError_t copy16(const void* src, void* dst, long long nwords, hipStream_t s);
+    static hipError_t copy_stream(const void* src, void* dst, long long nwords, hipStream_t s);
     static hipError_t diag(const Geo& g, const void* om, const void* rh, const int* jmp, const double* kx,
                            const double* ky, const double* sh, double* partial, int nblocks, double* out,
                            hipStream_t s);

@@ -441,6 +441,45 @@ shrt_status initialize_seeded(shrt_ctx* c, const shrt_seeded_ic* ic) {
 // ===========================================================================
 // C ABI
 // ===========================================================================
+// Average milliseconds of one device-to-device copy of `bytes` between two
+// fresh buffers: the plain grid-stride copy (the HBM-counter reference) or the
+// streaming copy (the achievable copy rate).
+static shrt_status copy_bench(long long bytes, int reps, bool stream, double* ms) {
+    if (bytes <= 0 || bytes % 16 != 0 || reps <= 0 || !ms) return fail(SHRT_ERR_INVALID, "bad arguments");
+    if (bytes / 16 > 1024LL * 0xffffffffLL) return fail(SHRT_ERR_INVALID, "copy too large");
+    void *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    shrt_status rc = SHRT_OK;
+    auto launch = [&]() {
+        return stream ? Launch<float>::copy_stream(a, b, bytes / 16, st) : Launch<float>::copy16(a, b, bytes / 16, st);
+    };
+    float t = 0.f;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+        rc = fail(SHRT_ERR_DEVICE, "hipStreamCreate failed");
+    } else if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) {
+        rc = fail(SHRT_ERR_ALLOC, "copy bench allocation failed");
+    } else if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
+               hipMemsetAsync(a, 1, bytes, st) != hipSuccess || launch() != hipSuccess ||
+               hipEventRecord(e0, st) != hipSuccess) {
+        rc = fail(SHRT_ERR_DEVICE, "copy bench set-up failed");
+    } else {
+        for (int i = 0; i < reps && rc == SHRT_OK; ++i)
+            if (launch() != hipSuccess) rc = fail(SHRT_ERR_DEVICE, "copy kernel launch failed");
+        if (rc == SHRT_OK && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                              hipEventElapsedTime(&t, e0, e1) != hipSuccess))
+            rc = fail(SHRT_ERR_DEVICE, "copy bench timing failed");
+    }
+    if (st) (void)hipStreamSynchronize(st);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (st) (void)hipStreamDestroy(st);
+    if (rc == SHRT_OK) *ms = t / reps;
+    return rc;
+}
+
 extern "C" {
 
 const char* shrt_last_error(void) { return g_last_error.c_str(); }
@@ -636,35 +675,9 @@ shrt_status shrt_time_steps(shrt_ctx* c, int nsteps, double* ms_total) {
     return shrt_synchronize(c);
 }
 
-shrt_status shrt_copy_bench(long long bytes, int reps, double* ms) {
-    if (bytes <= 0 || bytes % 16 != 0 || reps <= 0 || !ms) return fail(SHRT_ERR_INVALID, "bad arguments");
-    void *a = nullptr, *b = nullptr;
-    hipStream_t st = nullptr;
-    HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-    if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) {
-        if (a) (void)hipFree(a);
-        (void)hipStreamDestroy(st);
-        return fail(SHRT_ERR_ALLOC, "copy bench allocation failed");
-    }
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
-    HIP_TRY(hipMemsetAsync(a, 1, bytes, st));
-    HIP_TRY(Launch<float>::copy16(a, b, bytes / 16, st));   // warm
-    HIP_TRY(hipEventRecord(e0, st));
-    for (int i = 0; i < reps; ++i) HIP_TRY(Launch<float>::copy16(a, b, bytes / 16, st));
-    HIP_TRY(hipEventRecord(e1, st));
-    HIP_TRY(hipEventSynchronize(e1));
-    float t = 0.f;
-    HIP_TRY(hipEventElapsedTime(&t, e0, e1));
-    *ms = t / reps;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipFree(a);
-    (void)hipFree(b);
-    (void)hipStreamDestroy(st);
-    return SHRT_OK;
-}
+shrt_status shrt_copy_bench(long long bytes, int reps, double* ms) { return copy_bench(bytes, reps, false, ms); }
+
+shrt_status shrt_stream_copy_bench(long long bytes, int reps, double* ms) { return copy_bench(bytes, reps, true, ms); }
 
 }  // extern "C"
 

@@ -237,6 +237,14 @@ hipError_t Launch<R>::mul_acc(const void* a, const void* b, void* out, long long
 }
 
 template <>
+hipError_t Launch<R>::copy_stream(const void* src, void* dst, long long nwords, hipStream_t s) {
+    if (nwords <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_copy_stream<R>), dim3((unsigned)((nwords + 1023) / 1024)), dim3(256), 0, s,
+                       static_cast<const uint4*>(src), static_cast<uint4*>(dst), nwords);
+    return hipGetLastError();
+}
+
+template <>
 hipError_t Launch<R>::copy16(const void* src, void* dst, long long nwords, hipStream_t s) {
     hipLaunchKernelGGL((k_copy16<R>), dim3(4096), dim3(256), 0, s, static_cast<const uint4*>(src),
                        static_cast<uint4*>(dst), nwords);

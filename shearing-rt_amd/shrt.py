@@ -118,6 +118,7 @@ _SIGS = {
     "shrt_time_steps": ([_VP, ctypes.c_int, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "shrt_kernel_times": ([_VP, ctypes.c_int, _VP], ctypes.c_int),
     "shrt_copy_bench": ([ctypes.c_longlong, ctypes.c_int, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    "shrt_stream_copy_bench": ([ctypes.c_longlong, ctypes.c_int, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "shrt_sizes": ([ctypes.c_int, ctypes.c_int, _VP], ctypes.c_int),
     "shrt_supported": ([ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "shrt_device_count": ([], ctypes.c_int),
