@@ -247,8 +247,9 @@ def run_closure(shrt, args):
            "samples": int(len(x)), "data_s": t_data, "train_s": t_train, "train_loss_last": float(losses[-1]),
            "val_r2_best": p.val_best, "val_curve": p.val_curve,
            "error": {run: {"enstrophy_spectrum": mean(run, "enstrophy_spectrum"), "rho": mean(run, "rho")}
-                     for run in ("plain", "closure", "mid")},
-           "wall_s_per_run": {run: wall(run) for run in ("plain", "closure", "mid")},
+                     for run in ("plain", "closure", "closure_device", "mid")},
+           "wall_s_per_run": {run: wall(run) for run in ("plain", "closure", "closure_device", "mid")},
+           "device_vs_host_omega_rel_l2": float(max(r["device_vs_host"] for r in res)),
            "skipped_corrections": int(sum(r["skipped"] for r in res)), "per_seed": res}
     p.net.close()
     print(json.dumps(out), flush=True)

@@ -284,6 +284,18 @@ shrt_status shrt_closure_set_data(shrt_closure* net, const float* in, const floa
  * before its update. */
 shrt_status shrt_closure_train(shrt_closure* net, int iters, int batch, float lr, uint64_t seed, double* losses);
 
+/* In-place correction of a solver context by the network, all on the device
+ * (new; no reference counterpart).  Inputs: ktox of omega, rho and the stored
+ * phi of ctx (sheared frame), each divided by scale_in[i], and sigma as the
+ * fourth channel; the outputs times scale_out[j] are transformed with xtok and
+ * added, times dt_tot, to omega and rho (phi follows omega as in
+ * shrt_set_spectral).  Fallback: nothing changes, and *applied = 0, when an
+ * output or a correction is non-finite or |correction| * dt_tot > guard *
+ * |field| (L2 over the stored modes) for either field.  The network's grid must be ctx's (square).
+ * Same arithmetic as doing these steps through the host API. */
+shrt_status shrt_closure_correct(shrt_closure* net, shrt_ctx* ctx, const double* scale_in, double sigma,
+                                 const double* scale_out, double dt_tot, double guard, int* applied);
+
 /* Sizes derived by init_fft (fft.cu:91-96): out[0..7] = nkx, nky, nkxh, nkxh2,
  * nkxpad, ncy, nkx*nky, nkxh*ny. */
 shrt_status shrt_sizes(int nx, int ny, int* out8);

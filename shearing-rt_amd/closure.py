@@ -220,10 +220,12 @@ class Pipeline:
         n = self.cfg.n_coarse
         return [c.xtok((out[i] * self.scale_out[i]).reshape(n, n)) for i in range(2)]
 
-    def run_coarse(self, om, rh, t, sh, sigma, steps, closure=False, n=None):
+    def run_coarse(self, om, rh, t, sh, sigma, steps, closure=False, n=None, device=False):
         """A coarse (or, with n, another plain) run from a filtered state; returns
         the final (omega, rho) restricted to the coarse modes, the wall time and
-        the number of corrections skipped by the fallback."""
+        the number of corrections skipped by the fallback.  device=True applies
+        the corrections with shrt_closure_correct (no host round trip), the same
+        arithmetic as the host path below."""
         cfg, S = self.cfg, self.shrt
         n = n or cfg.n_coarse
         c = self.ctx(n, sigma)
@@ -246,7 +248,10 @@ class Pipeline:
         done = 0
         while done < steps:
             k = min(cfg.interval, steps - done)
-            if closure and k == cfg.interval:
+            if closure and device and k == cfg.interval:
+                if not self.net.correct(c, self.scale_in, sigma, self.scale_out, cfg.interval * cfg.delt, cfg.guard):
+                    skipped += 1
+            elif closure and k == cfg.interval:
                 corr = self.correction(c, sigma)
                 om_c, rh_c = c.spectral(S.OMEGA), c.spectral(S.RHO)
                 dt_tot = cfg.interval * cfg.delt
@@ -287,8 +292,11 @@ class Pipeline:
             ref = snaps[-1][:2]
             plain, w_plain, _ = self.run_coarse(om, rh, t, sh, sigma, steps)
             corr, w_corr, skipped = self.run_coarse(om, rh, t, sh, sigma, steps, closure=True)
+            dev, w_dev, skipped_dev = self.run_coarse(om, rh, t, sh, sigma, steps, closure=True, device=True)
             mid, w_mid, _ = self.run_coarse(om, rh, t, sh, sigma, steps, n=self.cfg.n_mid)
             res.append(dict(seed=seed, sigma=sigma, plain=self.errors(plain, ref), closure=self.errors(corr, ref),
-                            mid=self.errors(mid, ref), skipped=skipped,
-                            wall_s=dict(plain=w_plain, closure=w_corr, mid=w_mid)))
+                            closure_device=self.errors(dev, ref), mid=self.errors(mid, ref), skipped=skipped,
+                            skipped_device=skipped_dev,
+                            device_vs_host=float(np.linalg.norm(dev[0] - corr[0]) / np.linalg.norm(corr[0])),
+                            wall_s=dict(plain=w_plain, closure=w_corr, closure_device=w_dev, mid=w_mid)))
         return res

@@ -242,7 +242,7 @@ k_aux_yinv_lab(AuxArgs<T> a) {
 #pragma unroll
     for (int m = 0; m < P; ++m) {
         double sn, cs;
-        sincos(__dmul_rn(sh, (double)(tid + m * TT) * g.dy), &sn, &cs);
+        sincos(mul_rn(sh, (double)(tid + m * TT) * g.dy), &sn, &cs);
         v[m] = cmul(v[m], cplx<T>{(T)cs, (T)sn});
     }
 #pragma unroll
@@ -316,6 +316,62 @@ template <typename T>
 __global__ void k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, long long n) {
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
         dst[t] = src[t];
+}
+
+// Closure network input channel from a physical field: x = (float)(f / scale)
+// with the division in FP64 (what the host pipeline computes before its FP32
+// conversion); scale = 0 fills the channel with the constant `fill`.
+template <typename T>
+__global__ void k_net_in(const T* __restrict__ f, float* __restrict__ x, long long n, double scale, double fill) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    x[t] = scale != 0.0 ? (float)((double)f[t] / scale) : (float)fill;
+}
+
+// Physical correction field from a network output channel: f = (T)(out * scale).
+template <typename T>
+__global__ void k_net_out(const float* __restrict__ o, T* __restrict__ f, long long n, double scale) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) f[t] = (T)((double)o[t] * scale);
+}
+
+// One workgroup of 1024 threads: out[0] = sum |a|^2 over n complex values in
+// FP64, or (nonfinite) out[0] = the number of non-finite values among n floats.
+template <typename T>
+__global__ void __launch_bounds__(1024) k_sumsq(const cplx<T>* __restrict__ a, const float* __restrict__ o,
+                                                 long long n, double* __restrict__ out) {
+    __shared__ double red[1024];
+    double s = 0.0;
+    for (long long t = threadIdx.x; t < n; t += blockDim.x) {
+        if (a) {
+            const double x = (double)a[t].x, y = (double)a[t].y;
+            s += x * x + y * y;
+        } else {
+            s += isfinite(o[t]) ? 0.0 : 1.0;
+        }
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
+}
+
+// dst = a + (T)dt * c per component, each operation rounded in T (no
+// contraction): the host pipeline's state + dt * correction in the state's type.
+template <typename T>
+__global__ void k_axpy_round(const cplx<T>* __restrict__ a, const cplx<T>* __restrict__ c, double dt,
+                             cplx<T>* __restrict__ dst, long long n) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    if constexpr (sizeof(T) == 4) {
+        const float d = (float)dt;
+        dst[t] = {add_rn(a[t].x, mul_rn(d, c[t].x)), add_rn(a[t].y, mul_rn(d, c[t].y))};
+    } else {
+        dst[t] = {add_rn(a[t].x, mul_rn(dt, c[t].x)), add_rn(a[t].y, mul_rn(dt, c[t].y))};
+    }
 }
 
 // The fastest copy measured on MI355X (the achievable HBM rate): four 16-byte

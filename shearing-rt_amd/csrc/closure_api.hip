@@ -75,6 +75,8 @@ struct shrt_closure {
     int* idx = nullptr;
     long long adam_t = 0;
     std::vector<void*> allocs;
+    // ordering against a solver stream (closure_forward_on)
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
 };
 
 namespace {
@@ -241,6 +243,30 @@ hipError_t loss_and_grad(shrt_closure* n, int B) {
 
 }  // namespace
 
+namespace shrt {
+
+// Device-side access for the in-place correction of a solver context
+// (shrt_closure_correct): grid size, the input [4][N*N] and output [2][N*N]
+// buffers of sample 0, and one forward pass ordered after the work enqueued on
+// the solver stream `s`, with `s` ordered after it.
+int closure_grid(const shrt_closure* n) { return n->N; }
+float* closure_input(shrt_closure* n) { return n->x; }
+const float* closure_output(const shrt_closure* n) { return n->out; }
+
+hipError_t closure_forward_on(shrt_closure* n, hipStream_t s) {
+    hipError_t e = hipSuccess;
+    if (!n->ev_in) e = hipEventCreateWithFlags(&n->ev_in, hipEventDisableTiming);
+    if (e == hipSuccess && !n->ev_out) e = hipEventCreateWithFlags(&n->ev_out, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventRecord(n->ev_in, s);
+    if (e == hipSuccess) e = hipStreamWaitEvent(n->stream, n->ev_in, 0);
+    if (e == hipSuccess) e = forward(n, 1);
+    if (e == hipSuccess) e = hipEventRecord(n->ev_out, n->stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(s, n->ev_out, 0);
+    return e;
+}
+
+}  // namespace shrt
+
 extern "C" {
 
 shrt_status shrt_closure_create(int n_grid, int kmax, int width, int layers, int max_batch, uint64_t seed,
@@ -351,6 +377,8 @@ shrt_status shrt_closure_destroy(shrt_closure* n) {
     if (!n) return SHRT_OK;
     if (n->stream) (void)hipStreamSynchronize(n->stream);
     for (void* p : n->allocs) (void)hipFree(p);
+    if (n->ev_in) (void)hipEventDestroy(n->ev_in);
+    if (n->ev_out) (void)hipEventDestroy(n->ev_out);
     if (n->stream) (void)hipStreamDestroy(n->stream);
     delete n;
     return SHRT_OK;

@@ -36,6 +36,25 @@ template <int D, typename T> __device__ __forceinline__ cplx<T> cmul_i(cplx<T> a
     else return {a.y, -a.x};
 }
 
+// Separately rounded arithmetic that is never fused into an FMA.  The
+// toolchain defines __dadd_rn, __dmul_rn and the rest of that family as plain
+// operators, which -ffp-contract=fast contracts (a*b + c becomes one FMA);
+// these keep every operation rounded as written, for the FP64 shift
+// recurrence, kperp2, the CFL tests and the other places that follow the
+// reference's unfused arithmetic.
+#define SHRT_RN_OP(name, op)                                             \
+    __device__ __forceinline__ double name(double a, double b) {         \
+        _Pragma("clang fp contract(off)") return a op b;                 \
+    }                                                                    \
+    __device__ __forceinline__ float name(float a, float b) {            \
+        _Pragma("clang fp contract(off)") return a op b;                 \
+    }
+SHRT_RN_OP(add_rn, +)
+SHRT_RN_OP(sub_rn, -)
+SHRT_RN_OP(mul_rn, *)
+SHRT_RN_OP(div_rn, /)
+#undef SHRT_RN_OP
+
 // Compile-time loop: f(std::integral_constant<int, I>) for I in [B, E).
 template <int B, int E, typename F>
 __device__ __forceinline__ void static_for(F&& f) {

@@ -45,6 +45,11 @@ struct Launch {
                               hipStream_t s);
     static hipError_t copy16(const void* src, void* dst, long long nwords, hipStream_t s);
     static hipError_t copy_stream(const void* src, void* dst, long long nwords, hipStream_t s);
+    // closure correction helpers (shrt_closure_correct)
+    static hipError_t net_in(const void* f, float* x, long long n, double scale, double fill, hipStream_t s);
+    static hipError_t net_out(const float* o, void* f, long long n, double scale, hipStream_t s);
+    static hipError_t sumsq(const void* a, const float* o, long long n, double* out, hipStream_t s);
+    static hipError_t axpy_round(const void* a, const void* c, double dt, void* dst, long long n, hipStream_t s);
     static hipError_t diag(const Geo& g, const void* om, const void* rh, const int* jmp, const double* kx,
                            const double* ky, const double* sh, double* partial, int nblocks, double* out,
                            hipStream_t s);

@@ -47,6 +47,11 @@ constexpr int kDiagBlocks = 512;
 namespace shrt {
 void set_last_error(const std::string& msg) { g_last_error = msg; }
 thread_local LaunchTiming g_launch_timing;
+// closure_api.hip: device access to the network for shrt_closure_correct
+int closure_grid(const shrt_closure* n);
+float* closure_input(shrt_closure* n);
+const float* closure_output(const shrt_closure* n);
+hipError_t closure_forward_on(shrt_closure* n, hipStream_t s);
 }  // namespace shrt
 
 struct shrt_ctx {
@@ -1349,6 +1354,90 @@ shrt_status shrt_ensemble_time_steps(shrt_ensemble* e, int nsteps, double* ms_to
     if (st != SHRT_OK) return st;
     *ms_total = ms;
     return shrt_ensemble_synchronize(e);
+}
+
+}  // extern "C"
+
+namespace {
+
+template <typename T>
+shrt_status closure_correct_t(shrt_closure* net, shrt_ctx* c, const double* scale_in, double sigma,
+                              const double* scale_out, double dt_tot, double guard, int* applied) {
+    StepCtl v;
+    shrt_status st = read_ctl(c, &v);
+    if (st != SHRT_OK) return st;
+    const int slot = slot_a1(v.istep);
+    const double* s_now = c->s + (size_t)c->cur * c->g.nkx;
+    const long long NP = (long long)c->g.nx * c->g.ny, M = (long long)c->g.nkx * c->g.nky;
+    float* x = closure_input(net);
+    const float* out = closure_output(net);
+    // inputs: omega, rho, phi in physical space (phi as stored, through its view) / scale, then sigma
+    const void* src[3] = {c->ring[0][slot], c->ring[2][slot], c->aphi};
+    const int ep[3] = {v.slot_epoch_a[slot], v.slot_epoch_a[slot], v.phi_epoch};
+    for (int i = 0; i < 3; ++i) {
+        const int* jmp;
+        if ((st = view_for(c, v, ep[i], &jmp)) != SHRT_OK) return st;
+        HIP_TRY(ktox_dev(c, src[i], c->tmp_phys, s_now, jmp, 0));
+        HIP_TRY(Launch<T>::net_in(c->tmp_phys, x + i * NP, NP, scale_in[i], 0.0, c->stream));
+    }
+    HIP_TRY(Launch<T>::net_in(c->tmp_phys, x + 3 * NP, NP, 0.0, sigma, c->stream));
+    HIP_TRY(closure_forward_on(net, c->stream));
+    // corrections: xtok of out * scale_out, kept in G beyond ktox_dev's intermediate
+    void* corr[2] = {static_cast<char*>(c->G) + (size_t)c->g.I * c->esz,
+                     static_cast<char*>(c->G) + ((size_t)c->g.I + (size_t)M) * c->esz};
+    for (int i = 0; i < 2; ++i) {
+        HIP_TRY(Launch<T>::net_out(out + i * NP, c->tmp_phys, NP, scale_out[i], c->stream));
+        HIP_TRY(xtok_dev(c, c->tmp_phys, corr[i]));
+    }
+    // the state in the current frame, norms and the finiteness of the output
+    void* cur[2] = {c->tmp_spec, c->tmp_spec2};
+    for (int i = 0; i < 2; ++i) {
+        const int* jmp;
+        if ((st = view_for(c, v, v.slot_epoch_a[slot], &jmp)) != SHRT_OK) return st;
+        HIP_TRY(view_copy(c, c->ring[2 * i][slot], cur[i], jmp));
+    }
+    double* red = c->diag_partial;
+    HIP_TRY(Launch<T>::sumsq(corr[0], nullptr, M, red + 0, c->stream));
+    HIP_TRY(Launch<T>::sumsq(corr[1], nullptr, M, red + 1, c->stream));
+    HIP_TRY(Launch<T>::sumsq(cur[0], nullptr, M, red + 2, c->stream));
+    HIP_TRY(Launch<T>::sumsq(cur[1], nullptr, M, red + 3, c->stream));
+    HIP_TRY(Launch<T>::sumsq(nullptr, out, 2 * NP, red + 4, c->stream));
+    double r[5];
+    HIP_TRY(hipMemcpyAsync(r, red, sizeof r, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    // the fallback: a non-finite output or a correction larger than guard x the field
+    const bool ok = r[4] == 0.0 && std::isfinite(r[0]) && std::isfinite(r[1]) &&
+                    !(std::sqrt(r[0]) * dt_tot > guard * std::sqrt(r[2])) &&
+                    !(std::sqrt(r[1]) * dt_tot > guard * std::sqrt(r[3]));
+    *applied = ok ? 1 : 0;
+    if (!ok) return SHRT_OK;
+    for (int i = 0; i < 2; ++i)
+        HIP_TRY(Launch<T>::axpy_round(cur[i], corr[i], dt_tot, c->ring[2 * i][slot], M, c->stream));
+    // as shrt_set_spectral does for omega: phi of the new omega, both slots in the current frame
+    HIP_TRY(phi_from_omega(c, c->ring[0][slot], c->aphi, s_now));
+    v.phi_epoch = v.epoch;
+    v.slot_epoch_a[slot] = v.epoch;
+    return write_ctl(c, v);
+}
+
+}  // namespace
+
+extern "C" {
+
+shrt_status shrt_closure_correct(shrt_closure* net, shrt_ctx* c, const double* scale_in, double sigma,
+                                 const double* scale_out, double dt_tot, double guard, int* applied) {
+    if (!net || !c || !scale_in || !scale_out || !applied) return fail(SHRT_ERR_INVALID, "null argument");
+    *applied = 0;
+    if (closure_grid(net) != c->g.nx || c->g.nx != c->g.ny)
+        return fail(SHRT_ERR_INVALID, "the network's grid differs from the context's");
+    for (int i = 0; i < 3; ++i)
+        if (!(scale_in[i] > 0.0)) return fail(SHRT_ERR_INVALID, "input scales must be positive");
+    if (!std::isfinite(dt_tot) || !std::isfinite(guard) || !std::isfinite(sigma))
+        return fail(SHRT_ERR_INVALID, "non-finite dt, guard or sigma");
+    if (2 * (size_t)c->g.nkx * c->g.nky > 5 * (size_t)c->g.I)
+        return fail(SHRT_ERR_UNSUPPORTED, "correction scratch too small for this grid");
+    return c->prec == 4 ? closure_correct_t<float>(net, c, scale_in, sigma, scale_out, dt_tot, guard, applied)
+                        : closure_correct_t<double>(net, c, scale_in, sigma, scale_out, dt_tot, guard, applied);
 }
 
 }  // extern "C"

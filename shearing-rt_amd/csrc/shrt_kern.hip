@@ -237,6 +237,37 @@ hipError_t Launch<R>::mul_acc(const void* a, const void* b, void* out, long long
 }
 
 template <>
+hipError_t Launch<R>::net_in(const void* f, float* x, long long n, double scale, double fill, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_net_in<R>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, static_cast<const R*>(f), x,
+                       n, scale, fill);
+    return hipGetLastError();
+}
+
+template <>
+hipError_t Launch<R>::net_out(const float* o, void* f, long long n, double scale, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_net_out<R>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, o, static_cast<R*>(f), n,
+                       scale);
+    return hipGetLastError();
+}
+
+template <>
+hipError_t Launch<R>::sumsq(const void* a, const float* o, long long n, double* out, hipStream_t s) {
+    hipLaunchKernelGGL((k_sumsq<R>), dim3(1), dim3(1024), 0, s, static_cast<const cplx<R>*>(a), o, n, out);
+    return hipGetLastError();
+}
+
+template <>
+hipError_t Launch<R>::axpy_round(const void* a, const void* c, double dt, void* dst, long long n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_axpy_round<R>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                       static_cast<const cplx<R>*>(a), static_cast<const cplx<R>*>(c), dt, static_cast<cplx<R>*>(dst),
+                       n);
+    return hipGetLastError();
+}
+
+template <>
 hipError_t Launch<R>::copy_stream(const void* src, void* dst, long long nwords, hipStream_t s) {
     if (nwords <= 0) return hipSuccess;
     hipLaunchKernelGGL((k_copy_stream<R>), dim3((unsigned)((nwords + 1023) / 1024)), dim3(256), 0, s,

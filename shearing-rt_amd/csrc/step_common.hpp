@@ -245,10 +245,10 @@ __device__ __forceinline__ cplx<T> view_load(const cplx<T>* __restrict__ buf, in
 
 // Sheared ky and kperp2 in FP64 without contraction (shear.cu:335-336).
 __device__ __forceinline__ double kys_of(const double* ky, const double* s, int row, int iky) {
-    return __dadd_rn(ky[iky], s[row]);
+    return add_rn(ky[iky], s[row]);
 }
 __device__ __forceinline__ double k2_of(double kx, double kys) {
-    return __dadd_rn(__dmul_rn(kx, kx), __dmul_rn(kys, kys));
+    return add_rn(mul_rn(kx, kx), mul_rn(kys, kys));
 }
 
 // The step size after the CFL halvings of check_cfl (time_integral.cu:357-383).
@@ -261,13 +261,13 @@ __device__ __forceinline__ int cfl_delt(const StepCtl* c, const Geo& g, const Ph
     const double cr = ph.rho0_prime;
     int h = 0, h0 = 0, h1 = 0, h2 = 0;
     *which = -1;
-    while (__ddiv_rn(__dmul_rn(cvx, dt), g.dx) > ph.cfl_num && h < 2200) { dt /= 2.0; ++h; *which = 0; }
+    while (div_rn(mul_rn(cvx, dt), g.dx) > ph.cfl_num && h < 2200) { dt /= 2.0; ++h; *which = 0; }
     h0 = h;
-    while (__ddiv_rn(__dmul_rn(cvy, dt), g.dy) > ph.cfl_num && h < 2200) { dt /= 2.0; ++h; *which = 1; }
+    while (div_rn(mul_rn(cvy, dt), g.dy) > ph.cfl_num && h < 2200) { dt /= 2.0; ++h; *which = 1; }
     h1 = h;
-    while (__ddiv_rn(__dmul_rn(cg, dt), g.dx) > ph.cfl_num && h < 2200) { dt /= 2.0; ++h; *which = 2; }
+    while (div_rn(mul_rn(cg, dt), g.dx) > ph.cfl_num && h < 2200) { dt /= 2.0; ++h; *which = 2; }
     h2 = h;
-    while (__ddiv_rn(__dmul_rn(cr, dt), g.dx) > ph.cfl_num && h < 2200) { dt /= 2.0; ++h; *which = 3; }
+    while (div_rn(mul_rn(cr, dt), g.dx) > ph.cfl_num && h < 2200) { dt /= 2.0; ++h; *which = 3; }
     if (cnt) {
         cnt[0] = h0;
         cnt[1] = h1 - h0;

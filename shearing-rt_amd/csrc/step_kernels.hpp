@@ -46,7 +46,7 @@ struct ModeK {
 
 template <typename T>
 __device__ __forceinline__ ModeK<T> mode_k(const double* __restrict__ ky, double s_row, double kxv, int iky) {
-    const double kysd = __dadd_rn(ky[iky], s_row);
+    const double kysd = add_rn(ky[iky], s_row);
     ModeK<T> m;
     m.kx = (T)kxv;
     m.kys = (T)kysd;
@@ -448,7 +448,7 @@ k_p2(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
     // partial pair per workgroup (P3 reduces the array; no atomics)
     // (in the working precision, as add_v0 forms vx + sigma*yy in cureal; the
     // maxima are exact and converted to FP64 once per workgroup)
-    const T sy = (T)__dmul_rn(a.ph.sigma, (double)y * g.dy);
+    const T sy = (T)mul_rn(a.ph.sigma, (double)y * g.dy);
     T m0 = 0, m1 = 0;
 
     // Inverse x-transforms of G field pairs FA + i FB for this team's column.
@@ -652,8 +652,8 @@ __device__ __forceinline__ cplx<T> ab_bdf_update(int ist, bool plain, double dt,
         }
     }
     if (!plain) {                                              // eulerb / bdf2
-        const double cd = ist == 0 ? dt : __dmul_rn(2.0 / 3.0, dt);
-        const T den = (T)__dadd_rn(1.0, __dmul_rn(__dmul_rn(cd, diff), k2));
+        const double cd = ist == 0 ? dt : mul_rn(2.0 / 3.0, dt);
+        const T den = (T)add_rn(1.0, mul_rn(mul_rn(cd, diff), k2));
         r = cdivr(r, den);
     }
     return r;
@@ -742,14 +742,14 @@ template <typename T>
 __device__ void p3_bookkeeping(const StepArgs<T>& a, double dt, int h, int which, double cvx, double cvy) {
     const Geo& g = a.g;
     const StepCtl* c = a.ctl_in;
-    const double sdt1 = g.nkx > 1 ? __dmul_rn(__dmul_rn(a.ph.sigma, a.kx[1]), dt) : 0.0;
-    const double s1n = g.nkx > 1 ? __dsub_rn(a.s_in[1], sdt1) : 0.0;
-    const int trig = (g.nkx > 1 && fabs(__dsub_rn(s1n, -a.kx[1])) < 1e-10) ? 1 : 0;
+    const double sdt1 = g.nkx > 1 ? mul_rn(mul_rn(a.ph.sigma, a.kx[1]), dt) : 0.0;
+    const double s1n = g.nkx > 1 ? sub_rn(a.s_in[1], sdt1) : 0.0;
+    const int trig = (g.nkx > 1 && fabs(sub_rn(s1n, -a.kx[1])) < 1e-10) ? 1 : 0;
     const int newE = c->epoch + trig;
     int* jn = a.jump + (newE & 1) * g.nkx;
     for (int i = threadIdx.x; i < g.nkx; i += blockDim.x) {
         double sn = a.s_in[i];
-        if (i > 0) sn = __dsub_rn(a.s_in[i], __dmul_rn(__dmul_rn(a.ph.sigma, a.kx[i]), dt));
+        if (i > 0) sn = sub_rn(a.s_in[i], mul_rn(mul_rn(a.ph.sigma, a.kx[i]), dt));
         if (trig) {
             jn[i] = i == 0 ? 0 : (int)round(sn / a.kx[1]);
             if (i > 0) sn = 0.0;
@@ -760,7 +760,7 @@ __device__ void p3_bookkeeping(const StepArgs<T>& a, double dt, int h, int which
         // field by field into ctl_out (no private copy with dynamically indexed arrays)
         const long long n = c->istep;
         StepCtl* o = a.ctl_out;
-        o->time = __dadd_rn(c->time, dt);
+        o->time = add_rn(c->time, dt);
         o->delt = dt;
         o->istep = n + 1;
         o->epoch = newE;
@@ -915,7 +915,7 @@ k_p3(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
         const double kxv = kx_r[side];
         const double sr = s_r[side];
         // add_ddxrho: g/rho0 * kx;  add_ddxphi: rho0' * kx
-        const T cc = (T)__dmul_rn(f == 0 ? ph.g / ph.rho0 : ph.rho0_prime, kxv);
+        const T cc = (T)mul_rn(f == 0 ? ph.g / ph.rho0 : ph.rho0_prime, kxv);
 #pragma unroll
         for (int u = 0; u < JB; ++u) {
             const int iky = tid + (j0 + u) * TT;
@@ -925,7 +925,7 @@ k_p3(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
                 const cplx<T> F = side ? cconj(lds[(NY - iky) & (NY - 1)]) : lds[iky];
                 nh = {F.x * scale, F.y * scale};
             }
-            const double k2 = k2_of(kxv, __dadd_rn(B.ky[u], sr));
+            const double k2 = k2_of(kxv, add_rn(B.ky[u], sr));
             cplx<T> d0, an;
             if (f == 0) {
                 d0 = {nh.x + cc * B.x1[u].y, nh.y - cc * B.x1[u].x};
@@ -1016,16 +1016,16 @@ k_p3(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
         if (!act || (ph.linear && f == 1)) return;
         __syncthreads();   // the new rows are in LDS
         // shift and remap of this step for the two rows (p3_bookkeeping's operations)
-        const double sdt1 = g.nkx > 1 ? __dmul_rn(__dmul_rn(ph.sigma, a.kx[1]), dt) : 0.0;
-        const double s1n = g.nkx > 1 ? __dsub_rn(a.s_in[1], sdt1) : 0.0;
-        const bool trig = g.nkx > 1 && fabs(__dsub_rn(s1n, -a.kx[1])) < 1e-10;
+        const double sdt1 = g.nkx > 1 ? mul_rn(mul_rn(ph.sigma, a.kx[1]), dt) : 0.0;
+        const double s1n = g.nkx > 1 ? sub_rn(a.s_in[1], sdt1) : 0.0;
+        const bool trig = g.nkx > 1 && fabs(sub_rn(s1n, -a.kx[1])) < 1e-10;
         double sn[2];
         int jv[2];
 #pragma unroll
         for (int side = 0; side < 2; ++side) {
             const int r = side ? rn : row;
             double v = s_r[side];
-            if (r > 0) v = __dsub_rn(s_r[side], __dmul_rn(__dmul_rn(ph.sigma, kx_r[side]), dt));
+            if (r > 0) v = sub_rn(s_r[side], mul_rn(mul_rn(ph.sigma, kx_r[side]), dt));
             jv[side] = (trig && r > 0) ? (int)round(v / a.kx[1]) : 0;
             sn[side] = (trig && r > 0) ? 0.0 : v;
         }

@@ -107,6 +107,8 @@ _SIGS = {
     "shrt_closure_gradients": ([_VP, _VP, _VP, ctypes.c_int, ctypes.POINTER(ctypes.c_double), _VP], ctypes.c_int),
     "shrt_closure_set_data": ([_VP, _VP, _VP, ctypes.c_int], ctypes.c_int),
     "shrt_closure_train": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint64, _VP], ctypes.c_int),
+    "shrt_closure_correct": ([_VP, _VP, _VP, ctypes.c_double, _VP, ctypes.c_double, ctypes.c_double,
+                              ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "shrt_shearing_field": ([_VP, _VP, _VP, _VP], ctypes.c_int),
     "shrt_get_time": ([_VP, ctypes.POINTER(Time)], ctypes.c_int),
     "shrt_set_time": ([_VP, ctypes.c_int64, ctypes.c_double, ctypes.c_double], ctypes.c_int),
@@ -404,6 +406,17 @@ class Closure:
         losses = np.zeros(int(iters))
         _check(lib().shrt_closure_train(self.h, int(iters), int(batch), float(lr), int(seed), _ptr(losses)))
         return losses
+
+    def correct(self, ctx, scale_in, sigma, scale_out, dt_tot, guard):
+        """Correct the solver context `ctx` (a Shrt) in place on the device;
+        returns False when the fallback left it unchanged."""
+        si = np.ascontiguousarray(scale_in, dtype=np.float64)
+        so = np.ascontiguousarray(scale_out, dtype=np.float64)
+        assert si.shape == (3,) and so.shape == (2,)
+        applied = ctypes.c_int()
+        _check(lib().shrt_closure_correct(self.h, ctx.h, _ptr(si), float(sigma), _ptr(so), float(dt_tot),
+                                          float(guard), ctypes.byref(applied)))
+        return bool(applied.value)
 
     def close(self):
         if getattr(self, "h", None):

@@ -138,9 +138,83 @@ def test_closure_beats_the_plain_coarse_solver(gpu):
         print(f"  seed {r['seed']}: enstrophy spectrum error plain {r['plain']['enstrophy_spectrum']:.3f} "
               f"closure {r['closure']['enstrophy_spectrum']:.3f} plain 64^2 {r['mid']['enstrophy_spectrum']:.3f}; "
               f"rho plain {r['plain']['rho']:.3f} closure {r['closure']['rho']:.3f}")
-        assert r["skipped"] == 0
+        assert r["skipped"] == 0 and r["skipped_device"] == 0
+        # the device-resident correction makes the host path's arithmetic
+        assert r["device_vs_host"] < 1e-6, r["device_vs_host"]
     plain = np.mean([r["plain"]["enstrophy_spectrum"] for r in res])
     closure = np.mean([r["closure"]["enstrophy_spectrum"] for r in res])
     assert closure < 0.6 * plain
     assert np.mean([r["closure"]["rho"] for r in res]) < np.mean([r["plain"]["rho"] for r in res])
+    p.net.close()
+
+
+def _pipeline_module():
+    import importlib.util
+    here = os.path.dirname(os.path.abspath(__file__))
+    spec = importlib.util.spec_from_file_location("closure_pipe", os.path.join(here, "..", "shearing-rt_amd",
+                                                                               "closure.py"))
+    cl = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(cl)
+    return cl
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+def test_device_correction_matches_host(prec):
+    """shrt_closure_correct against the same correction made through the host
+    API (ktox of the stored fields, forward, xtok, set_spectral): the same
+    state afterwards, bit for bit, also right after a remap (pending view) and
+    over the steps that follow; the fallback (tiny guard, non-finite state)
+    leaves the state unchanged."""
+    cl = _pipeline_module()
+    n = 32
+    print(f"device closure correction prec={prec}", flush=True)
+    cfg = cl.Config(n_coarse=n, n_fine=64, n_mid=64, delt=1e-2, interval=10, cfl=5.0, width=8, kmax_net=6,
+                    layers=2, rho0_prime=-1.0)
+    p = cl.Pipeline(shrt, cfg)
+    p.net = shrt.Closure(n, 6, 8, 2, max_batch=1, seed=5)
+    p.scale_in = np.array([0.7, 0.3, 0.05])
+    p.scale_out = np.array([2.0, 0.5])
+    sigma, dt = 1.0, cfg.interval * cfg.delt
+
+    def make():
+        c = shrt.Shrt(nx=n, ny=n, precision=prec, graph_steps=0, **cfg.params(sigma))
+        c.initialize_seeded(seed=3, kmin=1.0, kmax=6.0, amp=0.3, expo=-1.0)
+        return c
+
+    def same(a, b):
+        for f in (shrt.OMEGA, shrt.RHO, shrt.PHI):
+            x, y = a.spectral(f), b.spectral(f)
+            assert np.array_equal(x.view(np.uint8), y.view(np.uint8)), f
+
+    a, b = make(), make()
+    for steps in (37, 63):   # the second run ends on the remap step (index 99)
+        a.time_advance(steps)
+        b.time_advance(steps)
+        corr = p.correction(a, sigma)
+        om, rh = a.spectral(shrt.OMEGA), a.spectral(shrt.RHO)
+        a.set_spectral(shrt.OMEGA, om + dt * corr[0])
+        a.set_spectral(shrt.RHO, rh + dt * corr[1])
+        assert p.net.correct(b, p.scale_in, sigma, p.scale_out, dt, 10.0)
+        same(a, b)
+        assert np.linalg.norm(a.spectral(shrt.OMEGA) - om) > 0
+    assert b.state()["remap_count"] == 1
+    a.time_advance(15)
+    b.time_advance(15)
+    same(a, b)
+    # fallback: a guard of zero rejects any correction
+    before = [b.spectral(f) for f in (shrt.OMEGA, shrt.RHO, shrt.PHI)]
+    assert not p.net.correct(b, p.scale_in, sigma, p.scale_out, dt, 0.0)
+    for f, x in zip((shrt.OMEGA, shrt.RHO, shrt.PHI), before):
+        assert np.array_equal(b.spectral(f).view(np.uint8), x.view(np.uint8))
+    # a non-finite state gives non-finite network outputs: no correction
+    bad = before[0].copy()
+    bad[1, 1] = np.nan
+    b.set_spectral(shrt.OMEGA, bad)
+    rho_before = b.spectral(shrt.RHO)
+    assert not p.net.correct(b, p.scale_in, sigma, p.scale_out, dt, 10.0)
+    assert np.array_equal(b.spectral(shrt.RHO).view(np.uint8), rho_before.view(np.uint8))
+    with pytest.raises(shrt.ShrtError):
+        p.net.correct(b, np.zeros(3), sigma, p.scale_out, dt, 10.0)
+    a.close()
+    b.close()
     p.net.close()
