@@ -489,3 +489,15 @@ def test_kernel_variants(gpu, prec):
     g.time_advance(steps)
     assert rel_l2(g.spectral(shrt.RHO), ref) < TOLN[prec]
     g.close()
+
+
+def test_copy_benches(gpu):
+    """The bandwidth references of the bench: both copy kernels time a copy and
+    reject sizes that are not whole 16-byte words."""
+    import ctypes
+    for fn in (shrt.lib().shrt_copy_bench, shrt.lib().shrt_stream_copy_bench):
+        ms = ctypes.c_double(-1.0)
+        assert fn(1 << 22, 3, ctypes.byref(ms)) == 0
+        assert 0.0 < ms.value < 100.0
+        assert fn(1000 * 16 + 8, 1, ctypes.byref(ms)) != 0
+        assert fn(0, 1, ctypes.byref(ms)) != 0
