@@ -1396,7 +1396,7 @@ shrt_status closure_correct_t(shrt_closure* net, shrt_ctx* c, const double* scal
         if ((st = view_for(c, v, v.slot_epoch_a[slot], &jmp)) != SHRT_OK) return st;
         HIP_TRY(view_copy(c, c->ring[2 * i][slot], cur[i], jmp));
     }
-    double* red = c->diag_partial;
+    double* red = c->diag_partial;   // the diagnostics' partial sums: free outside shrt_diagnostics
     HIP_TRY(Launch<T>::sumsq(corr[0], nullptr, M, red + 0, c->stream));
     HIP_TRY(Launch<T>::sumsq(corr[1], nullptr, M, red + 1, c->stream));
     HIP_TRY(Launch<T>::sumsq(cur[0], nullptr, M, red + 2, c->stream));
