@@ -198,13 +198,16 @@ __device__ __forceinline__ int xcd_item(int b, int n) {
 }
 
 // A complex value over a real one: two IEEE divisions in FP64 (the reference's
-// cmplx / real operator), one division and two products in FP32 (within 1.5 ulp).
+// cmplx / real operator); in FP32 the hardware reciprocal refined by one Newton
+// step and two products (within about 2 ulp; 0.6 µs less per 2048^2 step than
+// an IEEE division).
 template <typename T>
 __device__ __forceinline__ cplx<T> cdivr(cplx<T> z, T d) {
     if constexpr (sizeof(T) == 8) {
         return {z.x / d, z.y / d};
     } else {
-        const float r = 1.0f / d;
+        float r = __builtin_amdgcn_rcpf(d);
+        r = r * (2.0f - d * r);
         return {z.x * r, z.y * r};
     }
 }
