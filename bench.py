@@ -386,6 +386,7 @@ def main():
     g.set_fusion(args.fusion)
     if args.variants:
         g.set_variants(*[int(v) for v in args.variants.split(",")])
+    used = g.variants()
     g.initialize_seeded(**ic)
     if args.warmup > 0:
         g.time_advance(args.warmup)
@@ -431,7 +432,7 @@ def main():
         "config": {"workload": f"pseudospectral step, {n}^2 {args.precision.upper()} single instance "
                                "(sigma=1, nu=kappa=1e-4, g=1, delt=1e-4)",
                    "nx": n, "ny": n, "graph_steps": args.graph_steps, "fusion": args.fusion,
-                   "variants": args.variants or "default"},
+                   "variants": f"{used['p1']},{used['p2']},{used['p3']}" + ("" if args.variants else " (default)")},
         "roofline": {"bound": "hbm", "kernel": dname, "achieved": achieved, "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
                      "bytes_per_launch": kb[dname], "ms_per_launch": float(kt[dom])},

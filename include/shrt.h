@@ -213,7 +213,10 @@ shrt_status shrt_stream_copy_bench(long long bytes, int reps, double* ms);
  * compiled separately).  ms[] = average ms per launch of p1 row, p1 tile,
  * p1 pair, p2 three, p2 four, p3 pair, p3 tile.  It then times
  * captured batches of graph_steps steps (20 if the context has none) with and
- * without the P3+P1 fusion (shrt_set_fusion) and keeps the faster. */
+ * without the P3+P1 fusion (shrt_set_fusion) and keeps the faster.  Without
+ * autotuning a context starts with P1 variant 2 for ny >= 2048, 1 for ny = 512
+ * and for ny = 1024 in FP32, 0 otherwise (P2 0, P3 0, no fusion): the picks of
+ * shrt_autotune on MI355X.  shrt_get_variants reports the current choice. */
 typedef struct {
     int p1, p2, p3;
     double ms[7];
@@ -222,6 +225,7 @@ typedef struct {
 } shrt_tuning;
 shrt_status shrt_autotune(shrt_ctx* ctx, int reps, shrt_tuning* out);
 shrt_status shrt_set_variants(shrt_ctx* ctx, int p1, int p2, int p3);
+shrt_status shrt_get_variants(shrt_ctx* ctx, int* p1, int* p2, int* p3, int* fuse);
 
 /* Step fusion (new; no reference counterpart).  With on = 1 a captured batch
  * of graph_steps steps runs P3 of step n and P1 of step n+1 as one kernel: the

@@ -546,6 +546,11 @@ shrt_status shrt_create(const shrt_params* p, shrt_ctx** out) {
     Phys& ph = c->ph;
     ph.nu = p->nu; ph.kappa = p->kappa; ph.sigma = p->sigma; ph.g = p->g; ph.rho0 = p->rho0;
     ph.rho0_prime = p->rho0_prime; ph.cfl_num = p->cfl_num; ph.eps = 1e-10; ph.linear = p->linear ? 1 : 0;
+    // Default P1 variant by the y length, as shrt_autotune picks it on MI355X:
+    // one (row pair, field group) per workgroup from 2048 up, the four-row tile
+    // at 512 and at 1024 in FP32, one (row pair, output) otherwise.
+    if (g.ny >= 2048) c->var.p1 = kP1Pair;
+    else if (g.ny == 512 || (g.ny == 1024 && c->prec == 4)) c->var.p1 = kP1Tile;
 
     auto bail = [&](shrt_status st) {
         shrt_destroy(c);
@@ -790,6 +795,15 @@ shrt_status shrt_set_variants(shrt_ctx* c, int p1, int p2, int p3) {
             (void)hipGraphExecDestroy(c->gexec[i]);
             c->gexec[i] = nullptr;
         }
+    return SHRT_OK;
+}
+
+shrt_status shrt_get_variants(shrt_ctx* c, int* p1, int* p2, int* p3, int* fuse) {
+    if (!c || !p1 || !p2 || !p3 || !fuse) return fail(SHRT_ERR_INVALID, "null argument");
+    *p1 = c->var.p1;
+    *p2 = c->var.p2;
+    *p3 = c->var.p3;
+    *fuse = c->var.fuse;
     return SHRT_OK;
 }
 

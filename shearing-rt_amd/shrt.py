@@ -90,6 +90,8 @@ _SIGS = {
     "shrt_ktox_shear": ([_VP, _VP, _VP], ctypes.c_int),
     "shrt_autotune": ([_VP, ctypes.c_int, ctypes.POINTER(Tuning)], ctypes.c_int),
     "shrt_set_variants": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "shrt_get_variants": ([_VP, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                           ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "shrt_set_fusion": ([_VP, ctypes.c_int], ctypes.c_int),
     "shrt_ensemble_create": ([ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int,
                               ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
@@ -310,6 +312,12 @@ class Shrt:
 
     def set_variants(self, p1, p2, p3):
         _check(lib().shrt_set_variants(self.h, int(p1), int(p2), int(p3)))
+
+    def variants(self):
+        """The pass variants and fusion flag in use: dict(p1, p2, p3, fuse)."""
+        v = [ctypes.c_int() for _ in range(4)]
+        _check(lib().shrt_get_variants(self.h, *[ctypes.byref(x) for x in v]))
+        return dict(p1=v[0].value, p2=v[1].value, p3=v[2].value, fuse=v[3].value)
 
     def set_fusion(self, on):
         """Run captured batches with P3 of step n and P1 of step n+1 fused (default off)."""
