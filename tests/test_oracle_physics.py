@@ -147,6 +147,35 @@ def test_bracket_closed_form():
     np.testing.assert_allclose(o.physical("vy"), -phi_x, atol=1e-14)
 
 
+def test_bracket_closed_form_with_shift():
+    """The same Jacobian in the sheared frame: with ky_shift s(kx) = kx*s1 the y
+    derivatives take ky + s(kx) (ddy_shear) and phi = omega/(kx^2 + (ky + s)^2),
+    while the grid values keep the stored ky.  phi = A cos(x + 2y) with shift s1,
+    rho = C sin(3x + y) with shift 3*s1."""
+    n = 32
+    o = mk(n)
+    nkx, nky = o.shape_k
+    kx, _ = o.k_tables()
+    s1 = 0.3
+    o.set_shift(kx * s1)
+    x = np.arange(n) * TWO_PI / n
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    A, Cc = 0.3, 0.2
+    W = np.zeros((nkx, nky), complex)
+    W[1, 2] = (1.0 + (2.0 + s1) ** 2) * A / 2
+    o.set_spectral("omega", W)
+    o.check_cfl()
+    Rh = o.xtok(Cc * np.sin(3 * X + Y))
+    B = o.poisson_bracket(Rh)
+    vx = -(2.0 + s1) * A * np.sin(X + 2 * Y)
+    vy = A * np.sin(X + 2 * Y)
+    np.testing.assert_allclose(o.physical("vx"), vx, atol=1e-14)
+    np.testing.assert_allclose(o.physical("vy"), vy, atol=1e-14)
+    rho_x = 3 * Cc * np.cos(3 * X + Y)
+    rho_y = (1.0 + 3 * s1) * Cc * np.cos(3 * X + Y)
+    np.testing.assert_allclose(B, vx * rho_x + vy * rho_y, atol=1e-14)
+
+
 def test_remap_matches_physical_wavenumber_definition():
     """After a remap each stored mode holds the old mode with the same physical
     wavenumber (kx, ky + s(kx)), or the conjugate of its Hermitian partner, or 0."""
