@@ -176,6 +176,35 @@ def test_bracket_closed_form_with_shift():
     np.testing.assert_allclose(B, vx * rho_x + vy * rho_y, atol=1e-14)
 
 
+def test_diagnostics_are_domain_means():
+    """Energy, enstrophy and rho variance of the oracle (Hermitian-weighted sums
+    over the stored half plane) equal the domain means of |grad phi|^2/2
+    (sheared gradient), omega^2/2 and rho^2/2 of closed-form fields, including
+    a ky = 0 mode (stored twice, weight 1) and a ky shift."""
+    n = 32
+    o = mk(n)
+    nkx, nky = o.shape_k
+    kx, _ = o.k_tables()
+    s1 = -0.2
+    o.set_shift(kx * s1)
+    x = np.arange(n) * TWO_PI / n
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    A, Cc, D = 0.3, 0.5, 0.7
+    ks2 = 1.0 + (2.0 + s1) ** 2
+    W = np.zeros((nkx, nky), complex)
+    W[1, 2] = ks2 * A / 2                      # phi = A cos(x + 2y), omega = ks2 * phi
+    o.set_spectral("omega", W)
+    o.set_spectral("rho", o.xtok(Cc * np.cos(2 * X) + D * np.sin(X + 3 * Y)))
+    d = o.diagnostics()
+    phi_x = -A * np.sin(X + 2 * Y)
+    phi_y = -(2.0 + s1) * A * np.sin(X + 2 * Y)
+    omega = ks2 * A * np.cos(X + 2 * Y)
+    rho = Cc * np.cos(2 * X) + D * np.sin(X + 3 * Y)
+    assert d["energy"] == pytest.approx(np.mean(0.5 * (phi_x ** 2 + phi_y ** 2)), rel=1e-13)
+    assert d["enstrophy"] == pytest.approx(np.mean(0.5 * omega ** 2), rel=1e-13)
+    assert d["rho_var"] == pytest.approx(np.mean(0.5 * rho ** 2), rel=1e-13)
+
+
 def test_remap_matches_physical_wavenumber_definition():
     """After a remap each stored mode holds the old mode with the same physical
     wavenumber (kx, ky + s(kx)), or the conjugate of its Hermitian partner, or 0."""
