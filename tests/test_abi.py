@@ -77,3 +77,10 @@ def test_null_arguments():
     assert L.shrt_step(None, 1) == 1
     assert L.shrt_get_spectral(None, 0, None) == 1
     assert L.shrt_destroy(None) == 0
+    # entry points that check their arguments before touching the device
+    assert L.shrt_get_physical(None, 5, 0, None) == 1
+    assert L.shrt_get_variants(None, None, None, None, None) == 1
+    assert L.shrt_set_variants(None, 0, 0, 0) == 1
+    assert L.shrt_closure_correct(None, None, None, 1.0, None, 0.1, 1.0, None) == 1
+    assert L.shrt_copy_bench(0, 1, None) == 1
+    assert L.shrt_stream_copy_bench(24, 1, None) == 1
