@@ -55,6 +55,7 @@ def lib():
         L.orc_get_buffer.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
         L.orc_set_buffer.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
         L.orc_get_physical.argtypes = [vp, ctypes.c_int, dp]
+        L.orc_get_kdata_bef.argtypes = [vp, ctypes.c_int, dp]
         L.orc_step.argtypes = [vp, ctypes.c_int]
         L.orc_get_state.argtypes = [vp, dp]
         L.orc_set_time.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double]
@@ -150,6 +151,12 @@ class Oracle:
     def set_buffer(self, ring, slot, arr):
         a = np.ascontiguousarray(arr, dtype=np.complex128).reshape(-1)
         lib().orc_set_buffer(self.h, self.RING[ring], slot, _dp(a.view(np.float64)))
+
+    def kdata_bef(self, field):
+        """omega / phi / rho as update_shear handed them to k_data_bef at the last remap."""
+        out = np.zeros(self.s["M"], dtype=np.complex128)
+        lib().orc_get_kdata_bef(self.h, {"omega": 0, "phi": 1, "rho": 2}[field], _dp(out.view(np.float64)))
+        return out.reshape(self.shape_k)
 
     def physical(self, which):
         out = np.zeros(self.nx * self.ny, dtype=np.float64)

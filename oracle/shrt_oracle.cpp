@@ -137,6 +137,7 @@ struct Orc {
     int jump_flag;
     // fields.cu:18-25 (spectral state; ring slots rotate by swapping vectors)
     std::vector<C> aomg[3], domg[3], arho[3], drho[3], aphi;
+    std::vector<C> kdata_bef[3];   // omega, phi, rho as update_shear hands them to k_data_bef at the last remap
     std::vector<double> xx, yy, vx, vy, rtmp, ddx_r, ddy_r;
     // bookkeeping added by the restatement
     int istep;
@@ -513,6 +514,10 @@ void shearing_field(const Orc* o, const int* jmp, const C* in, C* out) {
 void update_shear(Orc* o) {
     o->jump_flag = shearing_ky(o);
     if (o->jump_flag) {
+        // k_data_bef(time+delt, istep+1) (shear.cu:236): the fields before the remap
+        o->kdata_bef[0] = o->aomg[0];
+        o->kdata_bef[1] = o->aphi;
+        o->kdata_bef[2] = o->arho[0];
         std::vector<C> tmp((size_t)o->nkx * o->nky);
         std::vector<C>* rings[4] = {o->aomg, o->domg, o->arho, o->drho};
         for (auto r : rings)
@@ -716,6 +721,15 @@ void orc_set_buffer(void* h, int ring, int slot, const double* in) {
         ring == 0 ? &o->aomg[slot] : ring == 1 ? &o->domg[slot] : ring == 2 ? &o->arho[slot]
         : ring == 3 ? &o->drho[slot] : &o->aphi;
     memcpy(dst->data(), in, sizeof(C) * dst->size());
+}
+
+// The fields k_data_bef wrote at the last remap (0 omega, 1 phi, 2 rho); zeros
+// before the first remap.
+void orc_get_kdata_bef(void* h, int field, double* out) {
+    Orc* o = static_cast<Orc*>(h);
+    const std::vector<C>& v = o->kdata_bef[field];
+    if (v.empty()) memset(out, 0, sizeof(C) * (size_t)o->nkx * o->nky);
+    else memcpy(out, v.data(), sizeof(C) * v.size());
 }
 
 void orc_get_physical(void* h, int which, double* out) {

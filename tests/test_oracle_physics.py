@@ -2,9 +2,12 @@
 mathematics fixes: closed-form solutions (linear Rayleigh-Taylor growth, viscous decay of a
 shearing wave), convergence order, conservation laws of the dealiased bracket, a closed-form
 Jacobian of single modes, the exact FP64 remap step indices, and a brute-force remap by
-physical wavenumber."""
+physical wavenumber.  The explicit weights of the AB2/AB3 updates in BDF2 form are
+pinned by the matrix exponential of the linear system with diffusion, and the sign of the
+advection term by a one-step closed-form tendency of trigonometric fields."""
 import numpy as np
 import pytest
+import scipy.linalg
 
 from conftest import oracle
 
@@ -266,3 +269,140 @@ def test_cfl_halving_count():
         k += 1
     assert st["halvings"] == k == 2
     assert st["delt"] == dt
+
+
+def _linear_exact(kx, ky, nu, kappa, g, rho0, rho0p, w0, r0, t):
+    """Exact solution of the linear (linear eq = true, sigma = 0) system for one mode:
+    d omega/dt = -(g/rho0) i kx rho - nu k^2 omega,  d rho/dt = rho0' i kx omega/k^2 - kappa k^2 rho
+    (add_ddxrho, add_ddxphi with phi = omega/k^2, and the implicit diffusion), from the
+    matrix exponential of its 2x2 generator."""
+    k2 = kx * kx + ky * ky
+    A = np.array([[-nu * k2, -(g / rho0) * 1j * kx],
+                  [rho0p * 1j * kx / k2, -kappa * k2]], dtype=complex)
+    return scipy.linalg.expm(A * t) @ np.array([w0, r0], dtype=complex)
+
+
+@pytest.mark.parametrize("nu,kappa", [(0.1, 0.1), (0.05, 0.0), (0.0, 0.08), (0.03, 0.12)])
+def test_linear_system_with_diffusion(nu, kappa):
+    """Every scheme branch with nonzero tendencies: Euler forward/backward at step 0,
+    AB2 and AB3 in BDF2 form with implicit BDF2 diffusion where the diffusivity is
+    positive (time_integral.cu:633-646, 678-695, 747-767), plain AB2/AB3 where it is zero
+    (:621-632, 664-677).  The buoyancy terms drive both fields, so the explicit weights act
+    on nonzero tendencies; the global error must be second order against the matrix
+    exponential.  A wrong weight (an inconsistent sum or a first-order start-up step)
+    leaves an O(1) or O(delt) error and fails the order check."""
+    kx, ky, eps, T = 1, 1, 1e-3, 1.0
+    errs = []
+    for dt in (2e-3, 1e-3):
+        o = mk(16, linear=1, delt=dt, nu=nu, kappa=kappa)
+        F = np.zeros(o.shape_k, complex)
+        F[kx, ky] = eps
+        o.set_spectral("rho", F)
+        o.step(int(round(T / dt)))
+        t = o.state()["time"]
+        w_ex, r_ex = _linear_exact(kx, ky, nu, kappa, 1.0, 1.0, 1.0, 0.0, eps, t)
+        w = o.buffer("aomg")[kx, ky]
+        r = o.buffer("arho")[kx, ky]
+        errs.append(max(abs(r - r_ex) / abs(r_ex), abs(w - w_ex) / abs(w_ex)))
+        # the other modes stay exactly zero
+        Wf = o.buffer("aomg").copy()
+        Wf[kx, ky] = 0
+        assert np.abs(Wf).max() == 0.0
+    assert errs[1] < 2e-5, errs
+    assert 3.3 < errs[0] / errs[1] < 4.7, errs   # second order
+
+
+def test_linear_system_unequal_modes_and_damping():
+    """The same closed form on two modes of different |k| at once, nu != kappa, with a
+    nonzero initial vorticity: the per-mode kperp2 of the implicit division and the kx
+    of the buoyancy terms are taken from the right rows."""
+    modes = [(1, 2), (3, 1)]
+    nu, kappa, T = 0.02, 0.05, 0.5
+    errs = []
+    for dt in (2e-3, 1e-3):
+        o = mk(16, linear=1, delt=dt, nu=nu, kappa=kappa, g=2.0, rho0=1.5, rho0_prime=0.7)
+        nkx = o.shape_k[0]
+        W = np.zeros(o.shape_k, complex)
+        R = np.zeros(o.shape_k, complex)
+        for (a, b) in modes:
+            W[a, b] = 1e-3 * (0.3 + 0.2j)
+            R[a, b] = 1e-3 * (1.0 - 0.5j)
+        # a kx < 0 row too: kx = -2 sits at row nkx - 2
+        W[nkx - 2, 3] = 2e-4j
+        R[nkx - 2, 3] = 5e-4
+        o.set_spectral("omega", W)
+        o.set_spectral("rho", R)
+        o.step(int(round(T / dt)))
+        t = o.state()["time"]
+        e = 0.0
+        for (row, kxv, b) in [(1, 1, 2), (3, 3, 1), (nkx - 2, -2, 3)]:
+            w_ex, r_ex = _linear_exact(kxv, b, nu, kappa, 2.0, 1.5, 0.7, W[row, b], R[row, b], t)
+            e = max(e, abs(o.buffer("aomg")[row, b] - w_ex) / abs(w_ex),
+                    abs(o.buffer("arho")[row, b] - r_ex) / abs(r_ex))
+        errs.append(e)
+    assert errs[1] < 2e-5, errs
+    assert 3.3 < errs[0] / errs[1] < 4.7, errs
+
+
+def test_one_step_tendencies_closed_form():
+    """The nonlinear tendencies of one step in closed form (time_integral.cu:488-617):
+    d omega/dt = -(vx d omega/dx + vy d omega/dy) - (g/rho0) i kx rho_hat,
+    d rho/dt   = -(vx d rho/dx + vy d rho/dy) + rho0' i kx phi_hat,
+    vx = d phi/dy, vy = -d phi/dx (get_vector_shear), for
+    phi = A cos(x + 2y) + D cos(3x + y) (omega = 5 A cos(x + 2y) + 10 D cos(3x + y)),
+    rho = C sin(3x + y) + E cos(x - 2y).  The products stay inside the 2/3 truncation on
+    32^2, so the spectra of the closed-form physical tendencies are exact.  Pins the sign
+    of the advection term (negative(), fourier.cu:170-176) and of both buoyancy terms;
+    the first step is Euler forward (nu = kappa = 0)."""
+    n = 32
+    dt = 1e-3
+    o = mk(n, linear=0, delt=dt, g=1.3, rho0=1.1, rho0_prime=0.6)
+    x = np.arange(n) * TWO_PI / n
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    A, D, C, E = 0.3, 0.1, 0.2, 0.15
+    phi = A * np.cos(X + 2 * Y) + D * np.cos(3 * X + Y)
+    om = 5 * A * np.cos(X + 2 * Y) + 10 * D * np.cos(3 * X + Y)
+    rho = C * np.sin(3 * X + Y) + E * np.cos(X - 2 * Y)
+    phi_x = -A * np.sin(X + 2 * Y) - 3 * D * np.sin(3 * X + Y)
+    phi_y = -2 * A * np.sin(X + 2 * Y) - D * np.sin(3 * X + Y)
+    om_x = -5 * A * np.sin(X + 2 * Y) - 30 * D * np.sin(3 * X + Y)
+    om_y = -10 * A * np.sin(X + 2 * Y) - 10 * D * np.sin(3 * X + Y)
+    rho_x = 3 * C * np.cos(3 * X + Y) - E * np.sin(X - 2 * Y)
+    rho_y = C * np.cos(3 * X + Y) + 2 * E * np.sin(X - 2 * Y)
+    vx, vy = phi_y, -phi_x
+    W0 = o.xtok(om)
+    R0 = o.xtok(rho)
+    P0 = o.xtok(phi)
+    o.set_spectral("omega", W0)
+    o.set_spectral("rho", R0)
+    np.testing.assert_allclose(o.buffer("aphi"), P0, atol=1e-15)
+    o.step(1)
+    kx, _ = o.k_tables()
+    ikx = 1j * kx[:, None]
+    dw_ex = o.xtok(-(vx * om_x + vy * om_y)) - (1.3 / 1.1) * ikx * R0
+    dr_ex = o.xtok(-(vx * rho_x + vy * rho_y)) + 0.6 * ikx * P0
+    # the bracket of omega is nonzero here (two modes of different |k|)
+    assert np.abs(o.xtok(vx * om_x + vy * om_y)).max() > 1e-3
+    np.testing.assert_allclose(o.buffer("domg"), dw_ex, atol=1e-14)
+    np.testing.assert_allclose(o.buffer("drho"), dr_ex, atol=1e-14)
+    np.testing.assert_allclose(o.buffer("aomg"), W0 + dt * dw_ex, atol=1e-15)
+    np.testing.assert_allclose(o.buffer("arho"), R0 + dt * dr_ex, atol=1e-15)
+
+
+def test_kdata_bef_snapshot_is_the_pre_remap_state():
+    """The oracle keeps the fields update_shear hands to k_data_bef at a remap
+    (shear.cu:236): remapped with the step's jump table they are the state after
+    the remap, and before any remap the snapshot is empty."""
+    n = 32
+    o = mk(n, sigma=1.0, delt=0.01, linear=0, nu=1e-2, kappa=1e-2, cfl_num=5.0)
+    o.set_spectral("rho", oracle.seeded_spectrum(n, n, TWO_PI, TWO_PI, 7, 1.0, 6.0, 1e-3, -1.0))
+    o.step(99)
+    assert np.abs(o.kdata_bef("rho")).max() == 0
+    o.step(1)
+    assert o.state()["last_remap_step"] == 99
+    _, jump = o.shift()
+    for f, ring in (("omega", "aomg"), ("phi", "aphi"), ("rho", "arho")):
+        bef = o.kdata_bef(f)
+        assert np.abs(bef).max() > 0
+        np.testing.assert_array_equal(o.shearing_field(jump, bef), o.buffer(ring))
+        assert not np.allclose(bef, o.buffer(ring))
