@@ -137,6 +137,14 @@ shrt_status shrt_synchronize(shrt_ctx* ctx);
 shrt_status shrt_get_spectral(shrt_ctx* ctx, int field, void* host);
 shrt_status shrt_set_spectral(shrt_ctx* ctx, int field, const void* host);
 
+/* The spectral fields as update_shear hands them to k_data_bef at a remap step
+ * (shear.cu:236, before the shearing_field calls of :238-268): right after a
+ * step that remapped, omega and rho of the new state and the aphi of that
+ * step, all in the frame the step started in; at any other time the same as
+ * shrt_get_spectral.  The remap itself is applied lazily: the stored buffers
+ * keep the old frame until the next step reads them through the remap. */
+shrt_status shrt_get_spectral_unremapped(shrt_ctx* ctx, int field, void* host);
+
 /* Ring access by the reference's names after the last rotation: slot 0 = *0,
  * 1 = *1, 2 = *2 (renew_fields, time_integral.cu:474-484). */
 shrt_status shrt_get_buffer(shrt_ctx* ctx, int ring, int slot, void* host);

@@ -5,14 +5,18 @@
 //   shrt [--config PATH] [--dble] [--graph-steps N] [--print-config]
 //
 // --dble selects FP64 (the reference's `make dble=on`); the default is FP32 as
-// in the reference's default build.  Between output times the driver runs
-// batches of steps on the device (one captured HIP graph per graph-steps
-// steps): delt only ever halves, so a batch of k steps with
-// time + (k+1)*delt below the next event cannot skip an event of the
-// reference loop.  The CFL lines the reference prints inside check_cfl
-// (time_integral.cu:361-383) are printed after each batch from the device's
-// halving log, in order.  An energy/enstrophy series is appended to energy.dat
-// at every field output (new; additive).
+// in the reference's default build.  The loop variables (time, the next
+// output times, tmax) have the build's precision as cureal does, so an FP32
+// run takes the reference's float-accumulated step count and file names.
+// Between events the driver runs batches of steps on the device (one captured
+// HIP graph per graph-steps steps); a batch ends on the step at which the
+// loop body could next act (an output time, the loop test, a remap), found by
+// replaying the loop's arithmetic with the current delt: delt only ever
+// halves, which moves every event later.  The CFL lines the reference prints
+// inside check_cfl (time_integral.cu:361-383) are printed after each batch
+// from the device's halving log, in order, which also gives each step's delt
+// for the time accumulation.  An energy/enstrophy series is appended to
+// energy.dat at every field output (new; additive).
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -122,6 +126,12 @@ struct Driver {
     std::vector<R> pbk;                    // advection term of the step about to run
     int printed_halvings = 0;
     bool energy_started = false;
+    // The loop variables of main.cu:14-20 in the build's precision (cureal):
+    // float in FP32 runs, where the reference accumulates time in float and so
+    // decides the step count, the output steps and the file names with it.
+    int istep = 0;
+    R time = 0, tmax = 0, next_output_time = 0, next_output_maxamp_time = 0;
+    double delt = 0;                       // exact in R: read as R, only ever halved
 
     Driver(shrt_ctx* x, const Config& cc, const Grid& gg) : ctx(x), c(cc), g(gg) {
         const size_t M = (size_t)g.nkx * g.nky, N = (size_t)g.nx * g.ny;
@@ -166,36 +176,87 @@ struct Driver {
         output_maxamp<R>(time, max_abs(omg.data(), N), max_abs(phi.data(), N), max_abs(rho.data(), N));
     }
 
+    void spectra_unremapped() {
+        check(shrt_get_spectral_unremapped(ctx, SHRT_OMEGA, ao.data()), "shrt_get_spectral_unremapped");
+        check(shrt_get_spectral_unremapped(ctx, SHRT_PHI, ap.data()), "shrt_get_spectral_unremapped");
+        check(shrt_get_spectral_unremapped(ctx, SHRT_RHO, ar.data()), "shrt_get_spectral_unremapped");
+    }
+
     void k_data_bef_now(double time, int istep) {
         spectra();
         k_data_bef<R>(g, time, istep, c.nwrite, ao.data(), ap.data(), ar.data());
     }
 
-    // the CFL lines of check_cfl for halvings logged since the last call
-    void print_halvings() {
+    // Steps to run before the loop body of main.cu can see an event, found by
+    // replaying the loop's own arithmetic (time += delt in R, the FP64 shift
+    // recurrence and remap test of shearing_ky) with the current delt.  A CFL
+    // halving inside the batch only shortens delt, which makes every event
+    // come later, never earlier.  Events: the loop test (time <= tmax), field
+    // and max-amplitude output times, the step before one whose bracket is
+    // written (advect_omg's time + delt > next_output_time), the k_data_bef of
+    // istep 1, and a remap (k_data_bef / k_data_aft inside update_shear).
+    long long plan(const std::vector<double>& shift) const {
+        const long long kcap = 1000000;
+        if (istep == 0) return 1;
+        const bool remaps = c.sigma != 0 && g.nkx > 1;
+        const double sdt1 = remaps ? (c.sigma * g.kx[1]) * delt : 0.0;
+        double s1 = remaps ? shift[1] : 0.0;
+        R t = time;
+        const R dt = (R)delt;
+        for (long long j = 0; j < kcap; ++j) {
+            t = t + dt;
+            if (remaps) {
+                s1 = s1 - sdt1;
+                if (std::fabs(s1 - (-g.kx[1])) < 1e-10) return j + 1;
+            }
+            if (!(t <= tmax)) return j + 1;
+            if (c.write_fields && (t > next_output_time || t > next_output_maxamp_time)) return j + 1;
+            if (c.write_fields && !c.linear_flag && t + dt > next_output_time) return j + 1;
+        }
+        return kcap;
+    }
+
+    // The step sizes of the steps [from, to) from the device's halving log:
+    // delt carries over from step to step and halves at the logged steps.
+    // Prints the CFL lines of check_cfl (time_integral.cu:361-383) with the
+    // time the loop holds at each of them, and returns the loop's time after
+    // the steps.
+    R advance_time(long long from, long long to, double& dt) {
         std::vector<shrt_halving> h(4096);
         int n = 0;
         check(shrt_get_halvings(ctx, h.data(), (int)h.size(), &n), "shrt_get_halvings");
         static const char* names[4] = {"cfl_vx", "cfl_vy", "cfl_(g/rho0)", "cfl_rho0_prime"};
-        for (int i = printed_halvings; i < n; ++i) {
-            double d = h[i].delt_before;
-            for (int k = 0; k < 4; ++k)
-                for (int j = 0; j < h[i].counts[k]; ++j) {
-                    d /= 2.0;
-                    std::printf("istep = %d, time = %g, %s = %g, delt = %g\n", (int)h[i].istep, h[i].time, names[k],
-                                h[i].values[k], d);
-                }
+        int next = printed_halvings;
+        R t = time;
+        for (long long i = from; i < to; ++i) {
+            if (next < n && h[next].istep == i) {
+                double d = h[next].delt_before;
+                for (int k = 0; k < 4; ++k)
+                    for (int j = 0; j < h[next].counts[k]; ++j) {
+                        d /= 2.0;
+                        std::printf("istep = %d, time = %g, %s = %g, delt = %g\n", (int)i, (double)t, names[k],
+                                    h[next].values[k], d);
+                    }
+                dt = d;
+                ++next;
+            }
+            t = t + (R)dt;
         }
-        printed_halvings = n;
+        printed_halvings = next;
+        return t;
     }
 
     int run() {
-        shrt_time t = now();
-        int istep = (int)t.istep;
-        double time = t.time;
-        double next_output_time = c.output_time;
-        const double output_maxamp_time = 0.1;
-        double next_output_maxamp_time = output_maxamp_time;
+        shrt_time st = now();
+        istep = (int)st.istep;
+        delt = st.delt;
+        time = (R)st.time;
+        tmax = (R)c.tmax;
+        next_output_time = (R)c.output_time;          // file_access.cu:195
+        const R output_maxamp_time = (R)0.1;          // main.cu:18-19
+        next_output_maxamp_time = output_maxamp_time;
+        int remaps_seen = st.remap_count;
+        std::vector<double> shift(g.nkx);
 
         std::printf("yy[ny] = %lf\n", g.yy[g.ny]);
         if (c.write_fields) {
@@ -208,39 +269,46 @@ struct Driver {
             k_data_aft<R>(g, time, istep, c.nwrite, ao.data(), ap.data(), ar.data());
         }
 
-        while (time <= c.tmax) {
-            // steps that cannot reach any event of the loop body
-            double horizon = c.tmax;
-            if (c.write_fields) {
-                horizon = std::fmin(horizon, next_output_time);
-                horizon = std::fmin(horizon, next_output_maxamp_time);
-            }
-            long long k = 1;
-            if (istep >= 1 && t.delt > 0) {
-                const double room = (horizon - time) / t.delt - 2.0;
-                if (room > 1.0) k = (long long)std::fmin(room, 1e6);
-            }
+        while (time <= tmax) {
             // advect_omg (time_integral.cu:500-505) writes the bracket it computes
             // when the step ends past the next output time; check_cfl can only
-            // shorten delt, so a step that may end there takes the bracket of the
-            // current state first (k is 1 there: the horizon keeps two steps).
+            // shorten delt, so a step that may end there runs alone and takes the
+            // bracket of the current state first.
             const bool pbk_now = c.write_fields && !c.linear_flag;
-            const bool pbk_due = pbk_now && time != 0 && time + t.delt > next_output_time;
+            const bool pbk_due = pbk_now && time != 0 && time + (R)delt > next_output_time;
             if (pbk_now && time == 0) output_pbk<R>(g, 0.0, pbk.data());
+            long long k = 1;
             if (pbk_due) {
-                k = 1;
                 check(shrt_get_physical(ctx, SHRT_P_BRACKET, 0, pbk.data()), "shrt_get_physical");
+            } else {
+                if (c.sigma != 0) check(shrt_get_shift(ctx, shift.data(), nullptr), "shrt_get_shift");
+                k = plan(shift);
             }
+            const long long from = istep;
             check(shrt_step(ctx, (int)k), "shrt_step");
-            t = now();
-            istep = (int)t.istep;
-            time = t.time;
-            print_halvings();
+            st = now();
+            istep = (int)st.istep;
+            time = advance_time(from, st.istep, delt);
             if (pbk_due && time > next_output_time) output_pbk<R>(g, time, pbk.data());
+
+            // update_shear's dumps around a remap (shear.cu:236, 270), with the
+            // time and step after the step; the batch ends on the remap step
+            if (st.remap_count != remaps_seen) {
+                if (st.remap_count == remaps_seen + 1 && st.last_remap_step == st.istep - 1 && c.write_fields) {
+                    spectra_unremapped();
+                    k_data_bef<R>(g, time, istep, c.nwrite, ao.data(), ap.data(), ar.data());
+                    spectra();
+                    k_data_aft<R>(g, time, istep, c.nwrite, ao.data(), ap.data(), ar.data());
+                } else if (c.write_fields) {
+                    std::fprintf(stderr, "shrt: remap at step %lld was not the last of its batch; befk/aftk not written\n",
+                                 (long long)st.last_remap_step);
+                }
+                remaps_seen = st.remap_count;
+            }
 
             if (istep == 1) k_data_bef_now(time, istep);
             if (c.write_fields && time > next_output_time) {
-                next_output_time += c.output_time;
+                next_output_time += (R)c.output_time;
                 output_fields_now(istep, time);
                 spectra();
                 en_spectral<R>(g, istep, time, ao.data(), ap.data(), ar.data());

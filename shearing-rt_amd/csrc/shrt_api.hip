@@ -62,6 +62,7 @@ struct shrt_ctx {
     Geo g{};
     Phys ph{};
     hipStream_t stream = nullptr;
+    int device = 0;          // the device current at creation (the stream's)
     void* ring[4][3] = {};   // a_omg, d_omg, a_rho, d_rho
     void* aphi = nullptr;
     void* G = nullptr;
@@ -260,6 +261,21 @@ hipError_t phi_from_omega(shrt_ctx* c, const void* om, void* phi, const double* 
 
 size_t spec_bytes(const shrt_ctx* c) { return (size_t)c->g.nkx * c->g.nky * c->esz; }
 size_t phys_bytes(const shrt_ctx* c) { return (size_t)c->g.nx * c->g.ny * c->rsz; }
+
+// The omega and rho buffers of an a-ring slot share one frame record, and so
+// do the two d-ring buffers of a slot.  Before one buffer of a slot is
+// overwritten in the current frame, its sibling is brought into that frame
+// too (the remap view applied in place through the scratch buffer), so that
+// marking the slot current does not drop the sibling's pending remap.
+shrt_status materialise(shrt_ctx* c, const StepCtl& v, int ring, int slot) {
+    const int ep = (ring == 0 || ring == 2) ? v.slot_epoch_a[slot] : v.slot_epoch_d[slot];
+    const int* jmp;
+    shrt_status st = view_for(c, v, ep, &jmp);
+    if (st != SHRT_OK || jmp == nullptr) return st;
+    HIP_TRY(view_copy(c, c->ring[ring][slot], c->tmp_spec, jmp));
+    HIP_TRY(hipMemcpyAsync(c->ring[ring][slot], c->tmp_spec, spec_bytes(c), hipMemcpyDeviceToDevice, c->stream));
+    return SHRT_OK;
+}
 
 // Physical slot of the reference's logical ring name (a0/a1/a2 after the last
 // rotation) when `n` steps are complete.
@@ -556,6 +572,7 @@ shrt_status shrt_create(const shrt_params* p, shrt_ctx** out) {
         shrt_destroy(c);
         return st;
     };
+    if (hipGetDevice(&c->device) != hipSuccess) return bail(fail(SHRT_ERR_DEVICE, "hipGetDevice failed"));
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(SHRT_ERR_DEVICE, "hipStreamCreate failed"));
     const size_t M = (size_t)g.nkx * g.nky;
@@ -877,16 +894,38 @@ shrt_status shrt_get_time(shrt_ctx* c, shrt_time* out) {
 
 shrt_status shrt_set_time(shrt_ctx* c, int64_t istep, double time, double delt) {
     if (!c || istep < 0 || !(delt > 0)) return fail(SHRT_ERR_INVALID, "shrt_set_time: bad arguments");
+    if (std::fabs(c->ph.sigma) * delt * 3.0 > 1.0)
+        return fail(SHRT_ERR_INVALID, "shrt_set_time: |sigma|*delt must be below 1/3 (remaps would overlap)");
     StepCtl v;
     shrt_status st = read_ctl(c, &v);
     if (st != SHRT_OK) return st;
-    // Keep the slot frames: slots are addressed by istep mod 3, so re-label them.
-    const int e = v.epoch;
+    // Ring slots are addressed by istep mod 3.  The reference's variables keep
+    // their buffers when only the counter changes, so when the new count
+    // addresses other slots, each logical slot's contents and frame move to
+    // the slot the new count gives it (a 3-cycle of copies through one
+    // scratch buffer per ring).
+    const long long n0 = v.istep;
+    if (((n0 - istep) % 3 + 3) % 3 != 0) {
+        int po[3], pn[3];
+        for (int j = 0; j < 3; ++j) { po[j] = logical_slot(n0, j); pn[j] = logical_slot(istep, j); }
+        int src_of[3];                                   // new slot pn[j] takes old slot po[j]
+        for (int j = 0; j < 3; ++j) src_of[pn[j]] = po[j];
+        const size_t bytes = spec_bytes(c);
+        for (int r = 0; r < 4; ++r) {
+            void** b = c->ring[r];
+            const int d0 = 0, d1 = src_of[d0], d2 = src_of[d1];   // a 3-cycle: src_of[d2] == d0
+            HIP_TRY(hipMemcpyAsync(c->tmp_spec, b[d0], bytes, hipMemcpyDeviceToDevice, c->stream));
+            HIP_TRY(hipMemcpyAsync(b[d0], b[d1], bytes, hipMemcpyDeviceToDevice, c->stream));
+            HIP_TRY(hipMemcpyAsync(b[d1], b[d2], bytes, hipMemcpyDeviceToDevice, c->stream));
+            HIP_TRY(hipMemcpyAsync(b[d2], c->tmp_spec, bytes, hipMemcpyDeviceToDevice, c->stream));
+        }
+        int ea[3], ed[3];
+        for (int k = 0; k < 3; ++k) { ea[k] = v.slot_epoch_a[src_of[k]]; ed[k] = v.slot_epoch_d[src_of[k]]; }
+        for (int k = 0; k < 3; ++k) { v.slot_epoch_a[k] = ea[k]; v.slot_epoch_d[k] = ed[k]; }
+    }
     v.istep = istep;
     v.time = time;
     v.delt = delt;
-    for (int s = 0; s < 3; ++s) { v.slot_epoch_a[s] = e; v.slot_epoch_d[s] = e; }
-    v.phi_epoch = e;
     return write_ctl(c, v);
 }
 
@@ -944,6 +983,24 @@ shrt_status shrt_get_spectral(shrt_ctx* c, int field, void* host) {
     return SHRT_OK;
 }
 
+shrt_status shrt_get_spectral_unremapped(shrt_ctx* c, int field, void* host) {
+    if (!c || !host) return fail(SHRT_ERR_INVALID, "null argument");
+    StepCtl v;
+    shrt_status st = read_ctl(c, &v);
+    if (st != SHRT_OK) return st;
+    const int slot = slot_a1(v.istep);
+    const void* src;
+    switch (field) {
+        case SHRT_OMEGA: src = c->ring[0][slot]; break;
+        case SHRT_RHO: src = c->ring[2][slot]; break;
+        case SHRT_PHI: src = c->aphi; break;
+        default: return fail(SHRT_ERR_INVALID, "unknown spectral field");
+    }
+    HIP_TRY(hipMemcpyAsync(host, src, spec_bytes(c), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return SHRT_OK;
+}
+
 shrt_status shrt_set_spectral(shrt_ctx* c, int field, const void* host) {
     if (!c || !host) return fail(SHRT_ERR_INVALID, "null argument");
     StepCtl v;
@@ -957,6 +1014,7 @@ shrt_status shrt_set_spectral(shrt_ctx* c, int field, const void* host) {
         case SHRT_PHI: dst = c->aphi; break;
         default: return fail(SHRT_ERR_INVALID, "unknown spectral field");
     }
+    if (field != SHRT_PHI && (st = materialise(c, v, field == SHRT_OMEGA ? 2 : 0, slot)) != SHRT_OK) return st;
     HIP_TRY(hipMemcpyAsync(dst, host, spec_bytes(c), hipMemcpyHostToDevice, c->stream));
     if (field == SHRT_OMEGA) {
         HIP_TRY(phi_from_omega(c, dst, c->aphi, c->s + (size_t)c->cur * c->g.nkx));
@@ -988,6 +1046,7 @@ shrt_status shrt_set_buffer(shrt_ctx* c, int ring, int j, const void* host) {
     shrt_status st = read_ctl(c, &v);
     if (st != SHRT_OK) return st;
     const int slot = logical_slot(v.istep, j);
+    if ((st = materialise(c, v, ring ^ 2, slot)) != SHRT_OK) return st;   // the sibling: a_omg<->a_rho, d_omg<->d_rho
     HIP_TRY(hipMemcpyAsync(c->ring[ring][slot], host, spec_bytes(c), hipMemcpyHostToDevice, c->stream));
     if (ring == 0 || ring == 2) v.slot_epoch_a[slot] = v.epoch;
     else v.slot_epoch_d[slot] = v.epoch;

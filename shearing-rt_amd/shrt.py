@@ -82,6 +82,7 @@ _SIGS = {
     "shrt_synchronize": ([_VP], ctypes.c_int),
     "shrt_get_spectral": ([_VP, ctypes.c_int, _VP], ctypes.c_int),
     "shrt_set_spectral": ([_VP, ctypes.c_int, _VP], ctypes.c_int),
+    "shrt_get_spectral_unremapped": ([_VP, ctypes.c_int, _VP], ctypes.c_int),
     "shrt_get_buffer": ([_VP, ctypes.c_int, ctypes.c_int, _VP], ctypes.c_int),
     "shrt_set_buffer": ([_VP, ctypes.c_int, ctypes.c_int, _VP], ctypes.c_int),
     "shrt_get_physical": ([_VP, ctypes.c_int, ctypes.c_int, _VP], ctypes.c_int),
@@ -224,6 +225,12 @@ class Shrt:
     def spectral(self, field):
         out = np.zeros(self.shape_k, dtype=self.cplx)
         _check(lib().shrt_get_spectral(self.h, field, _ptr(out)))
+        return out
+
+    def spectral_unremapped(self, field):
+        """The field as stored, before a pending remap (k_data_bef at a remap step)."""
+        out = np.zeros(self.shape_k, dtype=self.cplx)
+        _check(lib().shrt_get_spectral_unremapped(self.h, field, _ptr(out)))
         return out
 
     def set_spectral(self, field, arr):

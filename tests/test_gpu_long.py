@@ -1,20 +1,29 @@
-"""The plain method's GPU case at its full length (run on request: SHRT_LONG=1;
-about three minutes, most of it the FP64 oracle on the host).
+"""The plain method's GPU case at its full length, and the benchmark's 2048^2
+workload over 1000 steps (about three and one minutes, most of it the FP64
+oracle on the host).
 
 1024^2, Lx = Ly = 2 pi, S = 1 (remaps near steps 4000 and 8000), nu = kappa
 = 2e-4, g = 1, dt = 2.5e-4, 10000 steps, omega = 0 and a seeded random-phase
 rho on 2 <= |k| <= 32 with amplitude 1e-4 |k|^-1.5; FP64 and FP32 contexts
 against one FP64 oracle run, fields and the energy / enstrophy / rho-variance
-series compared every 1000 steps."""
+series compared every 1000 steps.  Each row of the comparison is printed and
+appended to gpurun_out/parity_long.txt as it is made."""
 import os
 
 import numpy as np
 import pytest
 
-from conftest import oracle, rel_l2, shrt
+from conftest import REPO, oracle, rel_l2, shrt
 
-pytestmark = [pytest.mark.gpu,
-              pytest.mark.skipif(not os.environ.get("SHRT_LONG"), reason="long run: set SHRT_LONG=1")]
+pytestmark = [pytest.mark.gpu]
+
+
+def _row(text):
+    print(text, flush=True)
+    d = os.path.join(REPO, "gpurun_out")
+    os.makedirs(d, exist_ok=True)
+    with open(os.path.join(d, "parity_long.txt"), "a") as f:
+        f.write(text + "\n")
 
 
 def test_gpu_case_10000_steps(gpu):
@@ -27,7 +36,7 @@ def test_gpu_case_10000_steps(gpu):
     gs = {prec: shrt.Shrt(nx=n, ny=n, precision=prec, graph_steps=20, **p) for prec in (8, 4)}
     for g in gs.values():
         g.initialize_seeded(**ic)
-    print("step | FP64 omega rho | FP32 omega rho | energy enstrophy rho_var (FP64 rel, FP32 rel) | remaps")
+    _row("1024^2 x 10000: step | FP64 omega rho | FP32 omega rho | energy enstrophy rho_var (FP64 rel, FP32 rel) | remaps")
     worst = {8: 0.0, 4: 0.0}
     for c in range(nchunk):
         o.step(chunk)
@@ -42,11 +51,11 @@ def test_gpu_case_10000_steps(gpu):
         row.append(" ".join(f"{k} {do[k]:.6e} ({abs(dd[8][k] - do[k]) / do[k]:.1e}, {abs(dd[4][k] - do[k]) / do[k]:.1e})"
                             for k in ("energy", "enstrophy", "rho_var")))
         row.append(str(o.state()["remap_count"]))
-        print(" | ".join(row), flush=True)
+        _row(" | ".join(row))
         for prec, g in gs.items():
             st = g.state()
             assert st["istep"] == (c + 1) * chunk and st["remap_count"] == o.state()["remap_count"]
-    print(f"worst field error FP64 {worst[8]:.2e}, FP32 {worst[4]:.2e}")
+    _row(f"worst field error FP64 {worst[8]:.2e}, FP32 {worst[4]:.2e}")
     assert worst[8] < 1e-10
     assert worst[4] < 1e-5
     for g in gs.values():
@@ -68,7 +77,7 @@ def test_headline_workload_1000_steps(gpu):
     gs = {prec: shrt.Shrt(nx=n, ny=n, precision=prec, graph_steps=20, **p) for prec in (8, 4)}
     for g in gs.values():
         g.initialize_seeded(**ic)
-    print("step | FP64 omega rho | FP32 omega rho | energy enstrophy rho_var (FP64 rel, FP32 rel)")
+    _row("2048^2 x 1000: step | FP64 omega rho | FP32 omega rho | energy enstrophy rho_var (FP64 rel, FP32 rel)")
     worst = {8: 0.0, 4: 0.0}
     for c in range(nchunk):
         o.step(chunk)
@@ -82,10 +91,10 @@ def test_headline_workload_1000_steps(gpu):
         dd = {prec: g.diagnostics() for prec, g in gs.items()}
         row.append(" ".join(f"{k} {do[k]:.6e} ({abs(dd[8][k] - do[k]) / do[k]:.1e}, {abs(dd[4][k] - do[k]) / do[k]:.1e})"
                             for k in ("energy", "enstrophy", "rho_var")))
-        print(" | ".join(row), flush=True)
+        _row(" | ".join(row))
         for g in gs.values():
             assert g.state()["istep"] == (c + 1) * chunk
-    print(f"worst field error FP64 {worst[8]:.2e}, FP32 {worst[4]:.2e}")
+    _row(f"worst field error FP64 {worst[8]:.2e}, FP32 {worst[4]:.2e}")
     assert worst[8] < 1e-10
     assert worst[4] < 1e-5
     for g in gs.values():

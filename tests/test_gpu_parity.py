@@ -56,11 +56,14 @@ def test_xtok_single_mode(gpu, prec):
 
 @pytest.mark.parametrize("prec", [8, 4])
 @pytest.mark.parametrize("istep", [0, 1, 2, 7])
-def test_one_step_random_state(gpu, prec, istep):
-    """One time_advance from a random sheared state, every scheme branch."""
+@pytest.mark.parametrize("nu,kappa", [(1e-3, 1e-3), (0.0, 1e-3), (1e-3, 0.0), (0.0, 0.0)])
+def test_one_step_random_state(gpu, prec, istep, nu, kappa):
+    """One time_advance from a random sheared state, every scheme branch: Euler,
+    AB2/AB3 in BDF2 form with implicit diffusion where the diffusivity is positive,
+    plain AB2/AB3 where it is zero (time_integral.cu:512-541, 575-605, 699-743)."""
     n = 64
-    print(f"one step prec={prec} istep={istep}", flush=True)
-    o, g = pair(n, prec, cfl_num=10.0)
+    print(f"one step prec={prec} istep={istep} nu={nu} kappa={kappa}", flush=True)
+    o, g = pair(n, prec, cfl_num=10.0, nu=nu, kappa=kappa)
     rng = np.random.default_rng(100 + istep)
     nkx, nky = o.shape_k
     g.set_time(istep, 0.1 * istep, 1e-3)
@@ -112,6 +115,35 @@ def test_multistep_seeded(gpu, prec, n, steps):
     so, sg = o.state(), g.state()
     assert sg["istep"] == so["istep"]
     assert abs(sg["time"] - so["time"]) < 1e-12
+
+
+@pytest.mark.parametrize("prec", [8, 4])
+@pytest.mark.parametrize("nu,kappa", [(0.0, 1e-3), (1e-3, 0.0), (0.0, 0.0)])
+def test_multistep_inviscid_branches(gpu, prec, nu, kappa):
+    """Many steps with zero viscosity and/or diffusivity (the plain AB2/AB3 updates with
+    no implicit division), single steps and fused captured batches, through a remap."""
+    n = 64
+    print(f"multistep inviscid prec={prec} nu={nu} kappa={kappa}", flush=True)
+    prm = dict(BASE, nu=nu, kappa=kappa, sigma=1.0, delt=2e-3)
+    ic = dict(seed=777, kmin=2.0, kmax=10.0, amp=2e-3, expo=-1.0)
+    steps = 600   # sigma*t crosses 1 at step 500: one remap
+    o = oracle.Oracle(nx=n, ny=n, **prm)
+    o.set_spectral("rho", oracle.seeded_spectrum(n, n, 2 * np.pi, 2 * np.pi, **ic))
+    o.step(steps)
+    assert o.state()["remap_count"] == 1
+    for gs in (0, 20):
+        g = shrt.Shrt(nx=n, ny=n, precision=prec, graph_steps=gs, **prm)
+        g.set_fusion(gs > 0)
+        g.initialize_seeded(**ic)
+        g.time_advance(steps)
+        for f, ring in ((shrt.OMEGA, "aomg"), (shrt.RHO, "arho"), (shrt.PHI, "aphi")):
+            err = rel_l2(g.spectral(f), o.buffer(ring))
+            print(f"  graph_steps={gs} field {f} rel err {err:.3e}")
+            assert err < TOLN[prec]
+        sg, so = g.state(), o.state()
+        assert sg["istep"] == so["istep"] and sg["remap_count"] == so["remap_count"] == 1
+        assert sg["last_remap_step"] == so["last_remap_step"]
+        g.close()
 
 
 @pytest.mark.parametrize("prec", [8, 4])
@@ -246,6 +278,155 @@ def test_remap_trigger(gpu, prec):
     g.time_advance(20)
     for f, ring in ((shrt.OMEGA, "aomg"), (shrt.RHO, "arho")):
         assert rel_l2(g.spectral(f), o.buffer(ring)) < TOLN[prec]
+
+
+def _remapped_pair(prec):
+    """Oracle and GPU context right after a remap step (index 99), every live
+    buffer of the GPU context still one frame behind (read through the view)."""
+    o, g = pair(32, prec, sigma=1.0, delt=0.01, cfl_num=5.0, nu=1e-2, kappa=1e-2)
+    ic = dict(seed=7, kmin=1.0, kmax=6.0, amp=1e-3, expo=-1.0)
+    o.set_spectral("rho", oracle.seeded_spectrum(32, 32, 2 * np.pi, 2 * np.pi, **ic))
+    g.initialize_seeded(**ic)
+    o.step(100)
+    g.time_advance(100)
+    assert o.state()["last_remap_step"] == g.state()["last_remap_step"] == 99
+    return o, g
+
+
+def _all_buffers_match(o, g, tol):
+    for ring in ("aomg", "arho", "domg", "drho"):
+        for slot in (0, 1, 2):
+            err = rel_l2(g.buffer(ring, slot), o.buffer(ring, slot))
+            assert err < tol, (ring, slot, err)
+    assert rel_l2(g.spectral(shrt.PHI), o.buffer("aphi")) < tol
+
+
+@pytest.mark.parametrize("prec", [8, 4])
+def test_spectral_before_remap(gpu, prec):
+    """Right after a remap step the stored buffers are the fields update_shear hands
+    to k_data_bef (shear.cu:236): the oracle's state with the remap of that step
+    undone, i.e. its buffers before shearing_field; the plain getters return the
+    remapped ones (k_data_aft)."""
+    print(f"spectral before remap prec={prec}", flush=True)
+    o, g = pair(32, prec, sigma=1.0, delt=0.01, cfl_num=5.0, nu=1e-2, kappa=1e-2)
+    ic = dict(seed=7, kmin=1.0, kmax=6.0, amp=1e-3, expo=-1.0)
+    o.set_spectral("rho", oracle.seeded_spectrum(32, 32, 2 * np.pi, 2 * np.pi, **ic))
+    g.initialize_seeded(**ic)
+    o.step(99)
+    g.time_advance(99)
+    for f in (shrt.OMEGA, shrt.RHO, shrt.PHI):     # no remap pending: both getters agree
+        assert np.array_equal(g.spectral_unremapped(f), g.spectral(f))
+    o.step(1)
+    g.time_advance(1)
+    assert o.state()["last_remap_step"] == g.state()["last_remap_step"] == 99
+    # the stored (unremapped) buffers, remapped with the step's jump table by the
+    # oracle's shearing_field, are the oracle's state after the remap
+    _, jump = o.shift()
+    for f, ring in ((shrt.OMEGA, "aomg"), (shrt.RHO, "arho"), (shrt.PHI, "aphi")):
+        raw = g.spectral_unremapped(f).astype(np.complex128)
+        assert rel_l2(raw, o.kdata_bef({shrt.OMEGA: "omega", shrt.RHO: "rho", shrt.PHI: "phi"}[f])) < TOLN[prec], f
+        fwd = o.shearing_field(jump, raw)
+        assert rel_l2(fwd, o.buffer(ring)) < TOLN[prec], f
+        assert rel_l2(g.spectral(f), o.buffer(ring)) < TOLN[prec], f
+        assert not np.allclose(raw, g.spectral(f))
+
+
+@pytest.mark.parametrize("prec", [8, 4])
+@pytest.mark.parametrize("new_istep", [100, 101, 105, 2])
+def test_set_time_keeps_state(gpu, prec, new_istep):
+    """shrt_set_time right after a remap, with a step count in each class mod 3 (the
+    ring slots are addressed by it): every named buffer keeps its contents and its
+    pending remap view, and the following steps match the oracle, whose set_time only
+    changes the counters.  A new step size beyond |sigma|*delt < 1/3 is refused."""
+    print(f"set_time prec={prec} istep={new_istep}", flush=True)
+    o, g = _remapped_pair(prec)
+    t = o.state()["time"]
+    o.set_time(new_istep, t, 0.005)
+    g.set_time(new_istep, t, 0.005)
+    _all_buffers_match(o, g, TOLN[prec])
+    st = g.state()
+    assert st["istep"] == new_istep and st["delt"] == 0.005
+    o.step(7)
+    g.time_advance(7)
+    for f, ring in ((shrt.OMEGA, "aomg"), (shrt.RHO, "arho"), (shrt.PHI, "aphi")):
+        assert rel_l2(g.spectral(f), o.buffer(ring)) < TOLN[prec]
+    with pytest.raises(RuntimeError):
+        g.set_time(new_istep, t, 0.34)
+
+
+@pytest.mark.parametrize("prec", [8, 4])
+@pytest.mark.parametrize("what", ["omega", "rho", "ring_aomg", "ring_drho"])
+def test_set_one_field_after_remap(gpu, prec, what):
+    """Right after a remap both fields of the state slot are pending; setting one of
+    them (or one buffer of a ring slot) must leave the other's remap in place.  The
+    next steps match the oracle given the same assignment."""
+    print(f"set one field after remap prec={prec} {what}", flush=True)
+    o, g = _remapped_pair(prec)
+    rng = np.random.default_rng(5)
+    arr = random_spectrum(rng, *o.shape_k, decay=2.0, amp=1e-3)
+    if prec == 4:
+        arr = arr.astype(np.complex64).astype(np.complex128)
+    if what in ("omega", "rho"):
+        o.set_spectral(what, arr)
+        g.set_spectral(shrt.OMEGA if what == "omega" else shrt.RHO, arr)
+    else:
+        ring = what[5:]
+        o.set_buffer(ring, 1, arr)
+        g.set_buffer(ring, 1, arr)
+    _all_buffers_match(o, g, TOLN[prec])
+    o.step(3)
+    g.time_advance(3)
+    for f, ring in ((shrt.OMEGA, "aomg"), (shrt.RHO, "arho")):
+        assert rel_l2(g.spectral(f), o.buffer(ring)) < TOLN[prec]
+
+
+@pytest.mark.parametrize("prec,n", [(8, 1024), (4, 1024), (8, 2048), (4, 2048), (4, 4096)])
+def test_remap_large_grid(gpu, prec, n):
+    """A remap on the large grids, where nkx - 1 exceeds 1024 from 2048^2 up (the size
+    at which the reference's per-block reset in shearing_ky races, shear.cu:275-294;
+    both sides implement its race-free reading).  The shift starts three steps short of
+    sigma*t = 1, so the remap fires on step index 2 with jumps up to nkx - 1 (rows
+    beyond nky empty, the conjugate fold for kx < 0): trigger step, shift and jump
+    table bit-exact, fields within tolerance before and after, single steps and fused
+    captured batches."""
+    print(f"large-grid remap n={n} prec={prec}", flush=True)
+    sigma, dt = 1.0, 1e-4
+    p = dict(BASE, sigma=sigma, nu=1e-4, kappa=1e-4, delt=dt)
+    o = oracle.Oracle(threads=16, nx=n, ny=n, **p)
+    kx, _ = o.k_tables()
+    s0 = -(kx * (1.0 - 3 * sigma * dt))
+    s0[0] = 0.0
+    ic = dict(seed=31 + n, kmin=2.0, kmax=32.0, amp=1e-4, expo=-1.5)
+    o.set_spectral("rho", oracle.seeded_spectrum(n, n, 2 * np.pi, 2 * np.pi, **ic))
+    o.set_shift(s0)
+    o.step(3)
+    so = o.state()
+    assert so["remap_count"] == 1 and so["last_remap_step"] == 2
+    sh_o, j_o = o.shift()
+    assert np.abs(j_o).max() == o.s["nkxh"] - 1
+    ref3 = {r: o.buffer(r) for r in ("aomg", "arho", "aphi")}
+    o.step(3)
+    for gs in (0, 2):
+        g = shrt.Shrt(nx=n, ny=n, precision=prec, graph_steps=gs, **p)
+        g.set_fusion(gs > 0)
+        g.initialize_seeded(**ic)
+        g.set_shift(s0)
+        g.time_advance(3)
+        sg = g.state()
+        assert sg["remap_count"] == 1 and sg["last_remap_step"] == 2
+        sh_g, j_g = g.shift()
+        assert np.array_equal(sh_o, sh_g) and np.array_equal(j_o, j_g)
+        for f, ring in ((shrt.OMEGA, "aomg"), (shrt.RHO, "arho"), (shrt.PHI, "aphi")):
+            err = rel_l2(g.spectral(f), ref3[ring])
+            print(f"  graph_steps={gs} after the remap step, field {f} rel err {err:.3e}", flush=True)
+            assert err < TOLN[prec]
+        g.time_advance(3)
+        for f, ring in ((shrt.OMEGA, "aomg"), (shrt.RHO, "arho")):
+            err = rel_l2(g.spectral(f), o.buffer(ring))
+            print(f"  graph_steps={gs} three steps later, field {f} rel err {err:.3e}", flush=True)
+            assert err < TOLN[prec]
+        g.close()
+    o.close()
 
 
 @pytest.mark.parametrize("prec", [8, 4])
