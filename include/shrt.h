@@ -162,6 +162,18 @@ shrt_status shrt_get_physical(shrt_ctx* ctx, int pfield, int lab_frame, void* ho
 shrt_status shrt_xtok(shrt_ctx* ctx, const void* phys_in, void* spec_out);
 shrt_status shrt_ktox(shrt_ctx* ctx, const void* spec_in, void* phys_out);
 
+/* Allocation-free xtok on device buffers.  No counterpart in the reference: for
+ * callers that cannot allocate device memory at the call (HIP graph capture,
+ * serving loops).  phys_dev (nx*ny reals) and spec_dev (nkx*nky complex) are the
+ * caller's device buffers; the result equals shrt_xtok's.  The x-forward
+ * intermediate (nkxh*ny complex) lives in a 16 MiB scratch array of the library
+ * (enough for 2048x2048 in FP32, 2048x1024 or 1024x2048 in FP64; a grid whose
+ * intermediate is larger is refused with SHRT_ERR_INVALID) that every GPU the library runs on holds until the process
+ * ends.  Enqueued on the context's stream without synchronising (pair with
+ * shrt_synchronize); no memset, no allocation, no host wait.  Calls that overlap
+ * on one GPU, from any contexts, share the scratch and race. */
+shrt_status shrt_xtok_noalloc(shrt_ctx* ctx, const void* phys_dev, void* spec_dev);
+
 /* ktox_shear (shear.cu:467-478) on a host buffer with the context's current
  * ky_shift: the lab-frame physical field of a spectral field. */
 shrt_status shrt_ktox_shear(shrt_ctx* ctx, const void* spec_in, void* phys_out);

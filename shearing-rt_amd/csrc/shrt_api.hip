@@ -15,6 +15,7 @@
 #include <cstring>
 #include <string>
 #include <map>
+#include <mutex>
 #include <vector>
 
 #include "../../include/shrt.h"
@@ -1125,6 +1126,77 @@ shrt_status shrt_xtok(shrt_ctx* c, const void* phys_in, void* spec_out) {
     HIP_TRY(xtok_dev(c, c->tmp_phys, c->tmp_spec));
     HIP_TRY(hipMemcpyAsync(spec_out, c->tmp_spec, spec_bytes(c), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    return SHRT_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Allocation-free xtok on device buffers
+// ---------------------------------------------------------------------------
+// The x-forward intermediate of shrt_xtok_noalloc (nkxh*ny complex values, every
+// element written by the x pass before the y pass reads it) lives in this
+// file-scope array instead of the context's H buffer.  Every GPU the library
+// runs on holds its own copy until the process ends; calls that overlap on one
+// GPU share it and race.  Host code never uses the array's name as an address:
+// each device's address comes from hipGetSymbolAddress, once per device.
+constexpr size_t kNoAllocScratchBytes = size_t(16) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, void*> g_scratch_ptr;   // device -> address of `scratch` on it
+
+// The address of `scratch` on device `dev`, which must be current.
+shrt_status scratch_ptr(int dev, void** out) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_ptr.find(dev);
+    if (it != g_scratch_ptr.end()) {
+        *out = it->second;
+        return SHRT_OK;
+    }
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr)
+        return fail(SHRT_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+    *out = p;
+    return SHRT_OK;
+}
+
+template <typename T>
+hipError_t xtok_scratch(shrt_ctx* c, const void* phys, void* spec, void* scr) {
+    AuxArgs<T> a = aux_args<T>(c, phys, scr, nullptr, nullptr, true, 0);
+    hipError_t e = Launch<T>::xfwd(a, c->stream);
+    if (e != hipSuccess) return e;
+    AuxArgs<T> b = aux_args<T>(c, scr, spec, nullptr, nullptr, false, 0);
+    return Launch<T>::yfwd(b, c->stream);
+}
+}  // namespace
+
+extern "C" {
+
+shrt_status shrt_xtok_noalloc(shrt_ctx* c, const void* phys_dev, void* spec_dev) {
+    if (!c || !phys_dev || !spec_dev) return fail(SHRT_ERR_INVALID, "null argument");
+    const size_t need = (size_t)c->g.I * c->esz;
+    if (need > kNoAllocScratchBytes)
+        return fail(SHRT_ERR_INVALID, "shrt_xtok_noalloc: a " + std::to_string(c->g.nx) + "x" +
+                                          std::to_string(c->g.ny) + " grid needs " + std::to_string(need) +
+                                          " bytes of scratch; the library's holds " +
+                                          std::to_string(kNoAllocScratchBytes) +
+                                          " (2048x2048 in FP32, 2048x1024 in FP64)");
+    int prev = 0;
+    HIP_TRY(hipGetDevice(&prev));
+    if (prev != c->device) HIP_TRY(hipSetDevice(c->device));
+    void* scr = nullptr;
+    shrt_status st = scratch_ptr(c->device, &scr);
+    hipError_t e = hipSuccess;
+    if (st == SHRT_OK)
+        e = c->prec == 4 ? xtok_scratch<float>(c, phys_dev, spec_dev, scr)
+                         : xtok_scratch<double>(c, phys_dev, spec_dev, scr);
+    if (prev != c->device) (void)hipSetDevice(prev);
+    if (st != SHRT_OK) return st;
+    HIP_TRY(e);
     return SHRT_OK;
 }
 

@@ -83,6 +83,7 @@ _SIGS = {
     "shrt_get_spectral": ([_VP, ctypes.c_int, _VP], ctypes.c_int),
     "shrt_set_spectral": ([_VP, ctypes.c_int, _VP], ctypes.c_int),
     "shrt_get_spectral_unremapped": ([_VP, ctypes.c_int, _VP], ctypes.c_int),
+    "shrt_xtok_noalloc": ([_VP, _VP, _VP], ctypes.c_int),
     "shrt_get_buffer": ([_VP, ctypes.c_int, ctypes.c_int, _VP], ctypes.c_int),
     "shrt_set_buffer": ([_VP, ctypes.c_int, ctypes.c_int, _VP], ctypes.c_int),
     "shrt_get_physical": ([_VP, ctypes.c_int, ctypes.c_int, _VP], ctypes.c_int),
