@@ -163,30 +163,48 @@ def cpu_scan(orc, params, ic, budget_s):
                               "cpus_available": avail}), flush=True)
 
 
-def run_ensemble(shrt, args):
-    """Aggregate throughput of independent instances stepped by one captured
-    graph per batch (shrt_ensemble_*).  SPEC is NxCOUNT,... or mixed:COUNT
+def ensemble_members(spec, seed=7):
+    """Parameters of the ensemble members: SPEC is NxCOUNT,... or mixed:COUNT
     (sizes drawn uniformly from 128, 256, 512).  Per instance: shear rate
-    U(0, 2), g U(0.5, 2), nu = kappa log-uniform in [1e-4, 1e-3], CFL number
-    0.3, rho seeded on 2 <= |k| <= n/16.  The autotuner picks the variants of
-    each grid size once (on that size's first instance) for all its members."""
-    rng = np.random.default_rng(7)
-    if args.ensemble.startswith("mixed:"):
-        sizes = [int(x) for x in rng.choice([128, 256, 512], int(args.ensemble.split(":")[1]))]
+    U(0, 2), g U(0.5, 2), nu = kappa log-uniform in [1e-4, 1e-3], initial delt
+    2e-4 under CFL number 0.3 (check_cfl halves it as needed), rho seeded on
+    2 <= |k| <= n/16 with its own seed."""
+    rng = np.random.default_rng(seed)
+    if spec.startswith("mixed:"):
+        sizes = [int(x) for x in rng.choice([128, 256, 512], int(spec.split(":")[1]))]
     else:
         sizes = []
-        for part in args.ensemble.split(","):
+        for part in spec.split(","):
             n, cnt = (int(x) for x in part.split("x"))
             sizes += [n] * cnt
-    members, points = [], 0
+    out = []
     for n in sizes:
         nu = float(10 ** rng.uniform(-4, -3))
-        g = shrt.Shrt(nx=n, ny=n, precision=shrt.FP32, graph_steps=0, Lx=2 * np.pi, Ly=2 * np.pi,
-                      delt=2e-4, cfl_num=0.3, nu=nu, kappa=nu, sigma=float(rng.uniform(0.0, 2.0)),
-                      g=float(rng.uniform(0.5, 2)), rho0=1.0, rho0_prime=1.0, rho_eps1=1e-2, noise=0, linear=0)
-        g.initialize_seeded(seed=int(rng.integers(1 << 62)), kmin=2.0, kmax=max(2.0, n / 16), amp=1e-4, expo=-1.5)
+        prm = dict(nx=n, ny=n, Lx=2 * np.pi, Ly=2 * np.pi, delt=2e-4, cfl_num=0.3, nu=nu, kappa=nu,
+                   sigma=float(rng.uniform(0.0, 2.0)), g=float(rng.uniform(0.5, 2)), rho0=1.0, rho0_prime=1.0,
+                   rho_eps1=1e-2, noise=0, linear=0)
+        ic = dict(seed=int(rng.integers(1 << 62)), kmin=2.0, kmax=max(2.0, n / 16), amp=1e-4, expo=-1.5)
+        out.append((prm, ic))
+    return out
+
+
+def run_ensemble(shrt, args):
+    """Aggregate throughput of independent instances stepped by one captured
+    graph per batch (shrt_ensemble_*), members from ensemble_members.  The
+    autotuner picks the variants of each grid size once (on that size's first
+    instance) for all its members.  The roofline fraction sums each member's
+    algorithmic bytes per step over the ensemble's time per step."""
+    members, points, sizes, nbytes, nbytes_c14 = [], 0, [], 0, 0
+    for prm, ic in ensemble_members(args.ensemble):
+        g = shrt.Shrt(precision=shrt.FP32, graph_steps=0, **prm)
+        g.initialize_seeded(**ic)
         members.append(g)
+        n = prm["nx"]
+        sizes.append(n)
         points += n * n
+        s = shrt.sizes(n, n)
+        nbytes += sum(kernel_bytes(s, 8).values())
+        nbytes_c14 += 8 * (14 * s["M"] + 10 * s["I"])
     tuned = {}
     for n in sorted(set(sizes)):
         first = members[sizes.index(n)]
@@ -205,6 +223,11 @@ def run_ensemble(shrt, args):
            "unit": "grid-pt*steps/s", "instances": len(members), "spec": args.ensemble, "steps": steps,
            "ms_per_step": ms / steps, "dtype": "fp32", "graph_steps": args.graph_steps,
            "sizes": {str(n): sizes.count(n) for n in sorted(set(sizes))},
+           "roofline": {"bound": "hbm", "bytes_per_step": nbytes, "unit": "GB/s",
+                        "achieved": nbytes / (ms / steps * 1e-3) / 1e9, "peak": PEAK_HBM_GBS,
+                        "frac": nbytes / (ms / steps * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                        "bytes_per_step_c14M10I": nbytes_c14,
+                        "frac_c14M10I": nbytes_c14 / (ms / steps * 1e-3) / 1e9 / PEAK_HBM_GBS},
            "tuned": {str(n): v for n, v in tuned.items()}}
     ens.close()
     for g in members:

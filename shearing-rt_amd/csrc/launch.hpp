@@ -18,7 +18,8 @@ extern thread_local LaunchTiming g_launch_timing;
 
 template <typename T>
 struct Launch {
-    static bool supported(int n);
+    static bool supported(int n);         // the default (one-team) step kernels fit length n
+    static bool tiles_supported(int n);   // the four-team kernels (P1/P3 tiles, the aux transforms) fit
     static int team_threads(int n);
     static int p2_workgroups(int nx, int ny);   // the P2 grid (number of CFL partials)
     // The step kernels.  dargs == nullptr: one instance, arguments `geom` passed

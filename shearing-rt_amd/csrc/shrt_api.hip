@@ -758,10 +758,16 @@ shrt_status shrt_autotune(shrt_ctx* c, int reps, shrt_tuning* out) {
     if (st == SHRT_OK) st = shrt_step(t, 4);
     static const int nvar[3] = {3, 2, 2};
     double ms[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    const bool tiles = c->prec == 4 ? Launch<float>::tiles_supported(c->g.ny) : Launch<double>::tiles_supported(c->g.ny);
     for (int pass = 0; pass < 3 && st == SHRT_OK; ++pass)
-        for (int v = 0; v < nvar[pass] && st == SHRT_OK; ++v)
+        for (int v = 0; v < nvar[pass] && st == SHRT_OK; ++v) {
+            if (!tiles && ((pass == 0 && v == kP1Tile) || (pass == 2 && v == kP3Tile))) {
+                ms[pass][v] = 1e300;   // four-team tiles do not fit this length's LDS
+                continue;
+            }
             st = c->prec == 4 ? time_pass<float>(t, pass, v, reps, &ms[pass][v])
                               : time_pass<double>(t, pass, v, reps, &ms[pass][v]);
+        }
     Variants chosen;
     chosen.p1 = kP1Row;
     for (int v = 1; v < nvar[0]; ++v)
@@ -805,6 +811,9 @@ shrt_status shrt_autotune(shrt_ctx* c, int reps, shrt_tuning* out) {
 
 shrt_status shrt_set_variants(shrt_ctx* c, int p1, int p2, int p3) {
     if (!c || p1 < 0 || p1 > 2 || p2 < 0 || p2 > 1 || p3 < 0 || p3 > 1) return fail(SHRT_ERR_INVALID, "bad variant");
+    const bool tiles = c->prec == 4 ? Launch<float>::tiles_supported(c->g.ny) : Launch<double>::tiles_supported(c->g.ny);
+    if (!tiles && (p1 == kP1Tile || p3 == kP3Tile))
+        return fail(SHRT_ERR_UNSUPPORTED, "the four-team tile variants do not fit this grid's LDS (FP64 4096)");
     c->var.p1 = p1;
     c->var.p2 = p2;
     c->var.p3 = p3;

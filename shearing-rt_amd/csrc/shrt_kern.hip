@@ -14,19 +14,21 @@ namespace {
 
 using R = SHRT_REAL;
 
-// Transform lengths compiled in.  A length fits when the four teams' LDS
-// buffers fit the 160 KiB of a CDNA4 workgroup.
-template <int N>
+// Transform lengths compiled in.  A kernel whose workgroups hold KT team
+// buffers of length N fits when they fit the 160 KiB of LDS a CDNA4 workgroup
+// can address: every length for one team (4096 in FP64 is 67.6 KB), all but
+// 4096 in FP64 for the four-team tile kernels.
+template <int N, int KT>
 constexpr bool fits() {
-    return (long long)kTile * lds_len<N>() * (long long)sizeof(cplx<R>) <= 160 * 1024;
+    return (long long)KT * lds_len<N>() * (long long)sizeof(cplx<R>) <= 160 * 1024;
 }
 
-template <typename F>
+template <int KT = 1, typename F>
 bool dispatch(int n, F&& f) {
     switch (n) {
 #define SHRT_CASE(N)                                              \
     case N:                                                       \
-        if constexpr (fits<N>()) {                                \
+        if constexpr (fits<N, KT>()) {                            \
             f(std::integral_constant<int, N>{});                  \
             return true;                                          \
         } else {                                                  \
@@ -67,6 +69,11 @@ bool Launch<R>::supported(int n) {
 }
 
 template <>
+bool Launch<R>::tiles_supported(int n) {
+    return dispatch<kTile>(n, [](auto) {});
+}
+
+template <>
 int Launch<R>::p2_workgroups(int nx, int ny) {
     int w = 0;
     dispatch(nx, [&](auto nc) { w = ny / p2_teams<R, decltype(nc)::value>(); });
@@ -82,6 +89,16 @@ int Launch<R>::team_threads(int n) {
 
 template <>
 hipError_t Launch<R>::p1(const StepArgs<R>& a, const StepArgs<R>* dargs, int count, int variant, hipStream_t s) {
+    if (variant == kP1Tile) {
+        const bool ok = dispatch<kTile>(a.g.ny, [&](auto nc) {
+            constexpr int N = decltype(nc)::value;
+            const P1Grid pg(a.g.nkxh, a.ph.linear);
+            const dim3 grid(pg.grid, dargs ? count : 1), block(kTile * TeamShape<N>::TT);
+            if (dargs) step_launch((k_p1<R, N, true>), grid, block, 0, s, a, dargs);
+            else step_launch((k_p1<R, N, false>), grid, block, 0, s, a, dargs);
+        });
+        return ok ? hipGetLastError() : unsupported();
+    }
     bool ok = dispatch(a.g.ny, [&](auto nc) {
         constexpr int N = decltype(nc)::value;
         if (variant == kP1Pair) {
@@ -92,11 +109,6 @@ hipError_t Launch<R>::p1(const StepArgs<R>& a, const StepArgs<R>* dargs, int cou
             const dim3 grid((a.g.nkxh - 1) * (a.ph.linear ? 2 : 6) + (a.ph.linear ? 1 : 3), dargs ? count : 1);
             if (dargs) step_launch((k_p1_row<R, N, true>), grid, dim3(TeamShape<N>::TT), 0, s, a, dargs);
             else step_launch((k_p1_row<R, N, false>), grid, dim3(TeamShape<N>::TT), 0, s, a, dargs);
-        } else {
-            const P1Grid pg(a.g.nkxh, a.ph.linear);
-            const dim3 grid(pg.grid, dargs ? count : 1), block(kTile * TeamShape<N>::TT);
-            if (dargs) step_launch((k_p1<R, N, true>), grid, block, 0, s, a, dargs);
-            else step_launch((k_p1<R, N, false>), grid, block, 0, s, a, dargs);
         }
     });
     return ok ? hipGetLastError() : unsupported();
@@ -107,6 +119,7 @@ hipError_t Launch<R>::p2(const StepArgs<R>& a, const StepArgs<R>* dargs, int cou
     bool ok = dispatch(a.g.nx, [&](auto nc) {
         constexpr int N = decltype(nc)::value;
         constexpr int KT = p2_teams<R, N>();
+        static_assert(KT == 1, "the P2 length dispatch assumes one team per workgroup");
         const dim3 grid(a.g.ny / KT, dargs ? count : 1), block(KT * Team<N, p2_points<R, N>()>::TT);
         if (variant == kP2Three) {
             if (dargs) step_launch((k_p2<R, N, true, true>), grid, block, 0, s, a, dargs);
@@ -121,17 +134,16 @@ hipError_t Launch<R>::p2(const StepArgs<R>& a, const StepArgs<R>* dargs, int cou
 
 template <>
 hipError_t Launch<R>::p3(const StepArgs<R>& a, const StepArgs<R>* dargs, int count, int variant, hipStream_t s) {
-    bool ok = dispatch(a.g.ny, [&](auto nc) {
-        constexpr int N = decltype(nc)::value;
-        auto go = [&](auto ktc) {
-            constexpr int KT = decltype(ktc)::value;
+    auto go = [&](auto ktc) {
+        constexpr int KT = decltype(ktc)::value;
+        return dispatch<KT>(a.g.ny, [&](auto nc) {
+            constexpr int N = decltype(nc)::value;
             const dim3 grid(2 * ((a.g.nkxh + KT - 1) / KT), dargs ? count : 1), block(KT * TeamShape<N>::TT);
             if (dargs) step_launch((k_p3<R, N, true, KT, false>), grid, block, 0, s, a, dargs);
             else step_launch((k_p3<R, N, false, KT, false>), grid, block, 0, s, a, dargs);
-        };
-        if (variant == kP3Pair) go(std::integral_constant<int, 1>{});
-        else go(std::integral_constant<int, kTile>{});
-    });
+        });
+    };
+    const bool ok = variant == kP3Pair ? go(std::integral_constant<int, 1>{}) : go(std::integral_constant<int, kTile>{});
     return ok ? hipGetLastError() : unsupported();
 }
 
@@ -148,7 +160,7 @@ hipError_t Launch<R>::p31(const StepArgs<R>& a, const StepArgs<R>* dargs, int co
 
 template <>
 hipError_t Launch<R>::xfwd(const AuxArgs<R>& a, hipStream_t s) {
-    bool ok = dispatch(a.g.nx, [&](auto nc) {
+    bool ok = dispatch<kTile>(a.g.nx, [&](auto nc) {
         constexpr int N = decltype(nc)::value;
         hipLaunchKernelGGL((k_aux_xfwd<R, N>), dim3(a.g.ntile_y), dim3(kTile * TeamShape<N>::TT), 0, s, a);
     });
@@ -157,7 +169,7 @@ hipError_t Launch<R>::xfwd(const AuxArgs<R>& a, hipStream_t s) {
 
 template <>
 hipError_t Launch<R>::yfwd(const AuxArgs<R>& a, hipStream_t s) {
-    bool ok = dispatch(a.g.ny, [&](auto nc) {
+    bool ok = dispatch<kTile>(a.g.ny, [&](auto nc) {
         constexpr int N = decltype(nc)::value;
         hipLaunchKernelGGL((k_aux_yfwd<R, N>), dim3(a.g.ntile_x), dim3(kTile * TeamShape<N>::TT), 0, s, a);
     });
@@ -166,7 +178,7 @@ hipError_t Launch<R>::yfwd(const AuxArgs<R>& a, hipStream_t s) {
 
 template <>
 hipError_t Launch<R>::yinv(const AuxArgs<R>& a, hipStream_t s) {
-    bool ok = dispatch(a.g.ny, [&](auto nc) {
+    bool ok = dispatch<kTile>(a.g.ny, [&](auto nc) {
         constexpr int N = decltype(nc)::value;
         hipLaunchKernelGGL((k_aux_yinv<R, N>), dim3(a.g.ntile_x), dim3(kTile * TeamShape<N>::TT), 0, s, a);
     });
@@ -175,7 +187,7 @@ hipError_t Launch<R>::yinv(const AuxArgs<R>& a, hipStream_t s) {
 
 template <>
 hipError_t Launch<R>::xinv(const AuxArgs<R>& a, hipStream_t s) {
-    bool ok = dispatch(a.g.nx, [&](auto nc) {
+    bool ok = dispatch<kTile>(a.g.nx, [&](auto nc) {
         constexpr int N = decltype(nc)::value;
         hipLaunchKernelGGL((k_aux_xinv<R, N>), dim3(a.g.ntile_y), dim3(kTile * TeamShape<N>::TT), 0, s, a);
     });
@@ -184,7 +196,7 @@ hipError_t Launch<R>::xinv(const AuxArgs<R>& a, hipStream_t s) {
 
 template <>
 hipError_t Launch<R>::yinv_lab(const AuxArgs<R>& a, hipStream_t s) {
-    bool ok = dispatch(a.g.ny, [&](auto nc) {
+    bool ok = dispatch<kTile>(a.g.ny, [&](auto nc) {
         constexpr int N = decltype(nc)::value;
         hipLaunchKernelGGL((k_aux_yinv_lab<R, N>), dim3((a.g.nkx + kTile - 1) / kTile),
                            dim3(kTile * TeamShape<N>::TT), 0, s, a);
@@ -194,7 +206,7 @@ hipError_t Launch<R>::yinv_lab(const AuxArgs<R>& a, hipStream_t s) {
 
 template <>
 hipError_t Launch<R>::xinv_lab(const AuxArgs<R>& a, hipStream_t s) {
-    bool ok = dispatch(a.g.nx, [&](auto nc) {
+    bool ok = dispatch<kTile>(a.g.nx, [&](auto nc) {
         constexpr int N = decltype(nc)::value;
         hipLaunchKernelGGL((k_aux_xinv_lab<R, N>), dim3(a.g.ntile_y), dim3(kTile * TeamShape<N>::TT), 0, s, a);
     });

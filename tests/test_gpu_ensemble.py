@@ -1,10 +1,17 @@
 """Ensembles: independent instances of mixed sizes, precisions, shear rates and
 seeds advanced by one captured graph per batch equal their individual runs
 bit for bit (including a remap inside the run)."""
+import importlib.util
+import os
+
 import numpy as np
 import pytest
 
-from conftest import shrt
+from conftest import REPO, shrt
+
+_spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(REPO, "bench.py"))
+bench = importlib.util.module_from_spec(_spec)
+_spec.loader.exec_module(bench)
 
 pytestmark = pytest.mark.gpu
 
@@ -60,3 +67,48 @@ def test_ensemble_rejects_bad_arguments(gpu):
             shrt.Ensemble([g, g], steps_per_graph=4)
     finally:
         g.close()
+
+
+def test_config4_mixed_ensemble_256(gpu):
+    """The batched-ensemble configuration at its full size: 256 FP32 instances
+    with grids drawn from {128, 256, 512}^2, shear U(0, 2), g U(0.5, 2),
+    nu = kappa log-uniform in [1e-4, 1e-3], delt 2e-4 under CFL number 0.3, rho
+    seeded on 2 <= |k| <= n/16 per instance (the draw of bench.py --ensemble
+    mixed:256), 5000 steps in captured graphs of 20 for the whole ensemble.
+    Every instance equals its own unbatched run (its own graphs of 20 steps)
+    bit for bit: the three spectral fields and the time, step, remap and
+    halving bookkeeping.  (With shear rates drawn at random the FP64 shift does
+    not land within 1e-10 of -kx[1], so, as in the reference, these runs never
+    remap; the remap inside an ensemble is covered above.)"""
+    steps = 5000
+    draw = bench.ensemble_members("mixed:256")
+    print(f"config-4 ensemble: {len(draw)} instances, sizes "
+          f"{ {n: sum(p['nx'] == n for p, _ in draw) for n in (128, 256, 512)} }", flush=True)
+    members = []
+    for prm, ic in draw:
+        g = shrt.Shrt(precision=shrt.FP32, graph_steps=0, **prm)
+        g.initialize_seeded(**ic)
+        members.append(g)
+    ens = shrt.Ensemble(members, steps_per_graph=20)
+    remapped = 0
+    try:
+        ens.step(steps)
+        print("ensemble stepped", flush=True)
+        for i, ((prm, ic), a) in enumerate(zip(draw, members)):
+            b = shrt.Shrt(precision=shrt.FP32, graph_steps=20, **prm)
+            b.initialize_seeded(**ic)
+            b.time_advance(steps)
+            for f in (shrt.OMEGA, shrt.RHO, shrt.PHI):
+                assert np.array_equal(a.spectral(f).view(np.uint32), b.spectral(f).view(np.uint32)), (i, prm, f)
+            sa, sb = a.state(), b.state()
+            assert sa == sb, (i, prm)
+            assert sa["istep"] == steps
+            remapped += sa["remap_count"] > 0
+            b.close()
+            if i % 32 == 31:
+                print(f"  {i + 1} instances equal to their unbatched runs", flush=True)
+        print(f"instances with a remap: {remapped}", flush=True)
+    finally:
+        ens.close()
+        for g in members:
+            g.close()

@@ -212,29 +212,64 @@ def test_small_case_full_length(gpu, prec):
     o.close()
 
 
-def test_large_grid_4096_fp32(gpu):
-    """The precision form at 4096^2 (FFT, bracket and update in FP32; shift,
-    time and the diagnostics' sums in FP64) for three steps from a seeded state
-    on 4 <= |k| <= 64 against the FP64 oracle, single steps and a captured
-    batch of two."""
+@pytest.mark.parametrize("prec", [4, 8])
+def test_large_grid_4096(gpu, prec):
+    """4096^2 for three steps from a seeded state on 4 <= |k| <= 64 against the
+    FP64 oracle, single steps and a captured batch of two: the precision form
+    (FFT, bracket and update in FP32; shift, time and the diagnostics' sums in
+    FP64) and the FP64 context (one-team kernels, 67.6 KB of LDS per team)."""
     n, steps = 4096, 3
-    print("4096^2 fp32 vs oracle", flush=True)
+    print(f"4096^2 prec={prec} vs oracle", flush=True)
     p = dict(BASE, sigma=1.5, nu=5e-5, kappa=5e-5, delt=5e-5)
     o = oracle.Oracle(threads=16, nx=n, ny=n, **p)
     ic = dict(seed=4096, kmin=4.0, kmax=64.0, amp=1e-4, expo=0.0)
     o.set_spectral("rho", oracle.seeded_spectrum(n, n, 2 * np.pi, 2 * np.pi, **ic))
     o.step(steps)
     for gs in (0, 2):
-        g = shrt.Shrt(nx=n, ny=n, precision=4, graph_steps=gs, **p)
+        g = shrt.Shrt(nx=n, ny=n, precision=prec, graph_steps=gs, **p)
         g.initialize_seeded(**ic)
         g.time_advance(steps)
         for f, ring in ((shrt.OMEGA, "aomg"), (shrt.RHO, "arho")):
             err = rel_l2(g.spectral(f), o.buffer(ring))
             print(f"  graph_steps={gs} field {f} rel err {err:.3e}")
-            assert err < TOLN[4]
+            assert err < TOLN[prec]
         assert g.state()["istep"] == steps
         g.close()
     o.close()
+
+
+def test_config3_precision_form_4096_2000_steps(gpu):
+    """The precision-form configuration at full length: 4096^2, sigma = 1.5,
+    nu = kappa = 5e-5, dt = 5e-5, 2000 steps, rho seeded on 4 <= |k| <= 64 with
+    amplitude 1e-4, omega = 0.  The FP32 context (FFT, bracket and update in
+    FP32; kx(t) phase, remap indices and the energy/enstrophy reductions in
+    FP64) against the FP64 GPU run of the same case: relative L2 of the fields
+    and relative difference of the compensated energy / enstrophy / rho
+    variance sums, every 500 steps; shift and step bookkeeping identical."""
+    n, chunk = 4096, 500
+    p = dict(BASE, sigma=1.5, nu=5e-5, kappa=5e-5, delt=5e-5)
+    ic = dict(seed=4096, kmin=4.0, kmax=64.0, amp=1e-4, expo=0.0)
+    gs = {prec: shrt.Shrt(nx=n, ny=n, precision=prec, graph_steps=20, **p) for prec in (8, 4)}
+    for g in gs.values():
+        g.initialize_seeded(**ic)
+    worst = 0.0
+    for c in range(4):
+        for g in gs.values():
+            g.time_advance(chunk)
+        e = [rel_l2(gs[4].spectral(f), gs[8].spectral(f)) for f in (shrt.OMEGA, shrt.RHO, shrt.PHI)]
+        d8, d4 = gs[8].diagnostics(), gs[4].diagnostics()
+        dd = {k: abs(d4[k] - d8[k]) / d8[k] for k in ("energy", "enstrophy", "rho_var")}
+        print(f"  step {(c + 1) * chunk}: FP32 vs FP64 GPU omega {e[0]:.2e} rho {e[1]:.2e} phi {e[2]:.2e}; "
+              f"energy {dd['energy']:.1e} enstrophy {dd['enstrophy']:.1e} rho_var {dd['rho_var']:.1e}", flush=True)
+        worst = max(worst, *e)
+        assert max(dd.values()) < 1e-5
+        s8, s4 = gs[8].state(), gs[4].state()
+        assert s8 == s4 and s8["istep"] == (c + 1) * chunk and s8["halvings"] == 0
+        assert np.array_equal(gs[8].shift()[0], gs[4].shift()[0])
+    print(f"  worst FP32 vs FP64 GPU field error after 2000 steps: {worst:.2e}", flush=True)
+    assert worst < 1e-5
+    for g in gs.values():
+        g.close()
 
 
 @pytest.mark.parametrize("prec", [8, 4])
