@@ -247,6 +247,18 @@ shrt_status shrt_autotune(shrt_ctx* ctx, int reps, shrt_tuning* out);
 shrt_status shrt_set_variants(shrt_ctx* ctx, int p1, int p2, int p3);
 shrt_status shrt_get_variants(shrt_ctx* ctx, int* p1, int* p2, int* p3, int* fuse);
 
+/* Precision of the FP32 state update (new; the reference's choice is its build
+ * flag -DDBLE).  wide = 1 (the default): the AB/BDF2 combination and the
+ * implicit-diffusion division (time_integral.cu:633-695, 747-767) are evaluated
+ * in FP64 from the FP32 operands and rounded once, stochastically (unbiased,
+ * deterministic: keyed by step, field and mode).  wide = 0: the same formulas in
+ * FP32 arithmetic rounded to nearest, with the coefficients the reference's FP32
+ * build forms in double ((2.0/3.0)*delt, delt/9.0, 1 + c*delt*nu*k^2), i.e. that
+ * build's arithmetic; it runs as its own P3 kernel, one row pair per workgroup,
+ * without the P3+P1 fusion (wide = 0 turns fusion off).  FP64 contexts ignore
+ * the flag.  Set it before an ensemble that holds the context is created. */
+shrt_status shrt_set_update_precision(shrt_ctx* ctx, int wide);
+
 /* Step fusion (new; no reference counterpart).  With on = 1 a captured batch
  * of graph_steps steps runs P3 of step n and P1 of step n+1 as one kernel: the
  * updated rows stay in LDS, the next shift and remap are formed in the kernel,

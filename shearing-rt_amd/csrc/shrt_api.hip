@@ -190,7 +190,7 @@ hipError_t enqueue_step(shrt_ctx* c, int in_par, int store_phi) {
 template <typename T>
 hipError_t enqueue_batch_step_t(shrt_ctx* c, int in_par, int k, int K, hipStream_t s) {
     const bool last = k == K - 1;
-    if (!c->var.fuse) {
+    if (!c->var.fuse || c->ph.narrow) {
         const StepArgs<T> a = step_args<T>(c, in_par, last);
         hipError_t e = Launch<T>::p1(a, nullptr, 1, c->var.p1, s);
         if (e == hipSuccess) e = Launch<T>::p2(a, nullptr, 1, c->var.p2, s);
@@ -563,6 +563,7 @@ shrt_status shrt_create(const shrt_params* p, shrt_ctx** out) {
     Phys& ph = c->ph;
     ph.nu = p->nu; ph.kappa = p->kappa; ph.sigma = p->sigma; ph.g = p->g; ph.rho0 = p->rho0;
     ph.rho0_prime = p->rho0_prime; ph.cfl_num = p->cfl_num; ph.eps = 1e-10; ph.linear = p->linear ? 1 : 0;
+    ph.narrow = 0;
     // Default P1 variant by the y length, as shrt_autotune picks it on MI355X:
     // one (row pair, field group) per workgroup from 2048 up, the four-row tile
     // at 512 and at 1024 in FP32, one (row pair, output) otherwise.
@@ -825,6 +826,18 @@ shrt_status shrt_set_variants(shrt_ctx* c, int p1, int p2, int p3) {
     return SHRT_OK;
 }
 
+shrt_status shrt_set_update_precision(shrt_ctx* c, int wide) {
+    if (!c || (wide != 0 && wide != 1)) return fail(SHRT_ERR_INVALID, "shrt_set_update_precision: bad arguments");
+    c->ph.narrow = (c->prec == 4 && !wide) ? 1 : 0;
+    if (c->ph.narrow) c->var.fuse = 0;   // its P3 instance runs unfused
+    for (int i = 0; i < 2; ++i)   // captured graphs hold the old arguments
+        if (c->gexec[i]) {
+            (void)hipGraphExecDestroy(c->gexec[i]);
+            c->gexec[i] = nullptr;
+        }
+    return SHRT_OK;
+}
+
 shrt_status shrt_get_variants(shrt_ctx* c, int* p1, int* p2, int* p3, int* fuse) {
     if (!c || !p1 || !p2 || !p3 || !fuse) return fail(SHRT_ERR_INVALID, "null argument");
     *p1 = c->var.p1;
@@ -836,6 +849,7 @@ shrt_status shrt_get_variants(shrt_ctx* c, int* p1, int* p2, int* p3, int* fuse)
 
 shrt_status shrt_set_fusion(shrt_ctx* c, int on) {
     if (!c || on < 0 || on > 1) return fail(SHRT_ERR_INVALID, "bad fusion flag");
+    if (on && c->ph.narrow) return fail(SHRT_ERR_INVALID, "the FP32-arithmetic update (shrt_set_update_precision 0) runs unfused");
     c->var.fuse = on;
     for (int i = 0; i < 2; ++i)
         if (c->gexec[i]) {
@@ -1415,7 +1429,9 @@ shrt_status shrt_ensemble_create(shrt_ctx* const* members, int n, int steps_per_
         size_t gi = 0;
         for (; gi < e->groups.size(); ++gi) {
             const shrt_ctx* r = members[e->groups[gi][0]];
-            if (r->prec == c->prec && r->g.nx == c->g.nx && r->g.ny == c->g.ny && r->ph.linear == c->ph.linear) break;
+            if (r->prec == c->prec && r->g.nx == c->g.nx && r->g.ny == c->g.ny && r->ph.linear == c->ph.linear &&
+                r->ph.narrow == c->ph.narrow)
+                break;
         }
         if (gi == e->groups.size()) e->groups.emplace_back();
         e->groups[gi].push_back(i);

@@ -143,12 +143,22 @@ hipError_t Launch<R>::p3(const StepArgs<R>& a, const StepArgs<R>* dargs, int cou
             else step_launch((k_p3<R, N, false, KT, false>), grid, block, 0, s, a, dargs);
         });
     };
+    if (sizeof(R) == 4 && a.ph.narrow) {   // the FP32-arithmetic update: one row pair per workgroup
+        const bool ok = dispatch(a.g.ny, [&](auto nc) {
+            constexpr int N = decltype(nc)::value;
+            const dim3 grid(2 * a.g.nkxh, dargs ? count : 1), block(TeamShape<N>::TT);
+            if (dargs) step_launch((k_p3<R, N, true, 1, false, true>), grid, block, 0, s, a, dargs);
+            else step_launch((k_p3<R, N, false, 1, false, true>), grid, block, 0, s, a, dargs);
+        });
+        return ok ? hipGetLastError() : unsupported();
+    }
     const bool ok = variant == kP3Pair ? go(std::integral_constant<int, 1>{}) : go(std::integral_constant<int, kTile>{});
     return ok ? hipGetLastError() : unsupported();
 }
 
 template <>
 hipError_t Launch<R>::p31(const StepArgs<R>& a, const StepArgs<R>* dargs, int count, hipStream_t s) {
+    if (a.ph.narrow) return unsupported();   // the FP32-arithmetic update runs unfused
     bool ok = dispatch(a.g.ny, [&](auto nc) {
         constexpr int N = decltype(nc)::value;
         const dim3 grid(2 * a.g.nkxh, dargs ? count : 1), block(TeamShape<N>::TT);

@@ -82,7 +82,9 @@ struct Geo {
 struct Phys {
     double nu, kappa, sigma, g, rho0, rho0_prime, cfl_num, eps;
     int linear;
-    int pad_;
+    int narrow;   // FP32 contexts: 1 = the state update in FP32 arithmetic, rounded to nearest (the
+                  // reference's FP32 build; its own P3 instance); 0 = evaluated in FP64 and rounded
+                  // once, stochastically
 };
 
 // Device-resident control block.  Step n reads ctl_in and its last kernel

@@ -814,7 +814,7 @@ __device__ void p3_bookkeeping(const StepArgs<T>& a, double dt, int h, int which
 // the same FP64 operations as the bookkeeping, and the outputs of this field
 // (omega: vx, vy, d/dx, d/dy; rho: d/dx, d/dy) are transformed and written to
 // G through the remap view, as k_p1_row would after reading the rows back.
-template <typename T, int NY, bool BATCH, int KT, bool FUSE>
+template <typename T, int NY, bool BATCH, int KT, bool FUSE, bool NARROW = false>
 __global__ void __launch_bounds__((KT * TeamShape<NY>::TT), (waves_per_eu<T, NY, TeamShape<NY>::TT, KT>()))
 k_p3(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
     const StepArgs<T>& a = step_args_of<BATCH>(a1, args);
@@ -933,7 +933,7 @@ k_p3(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
                 const cplx<T> phi = (r == 0 && iky == 0) ? czero<T>() : cdivr(B.x1[u], (T)k2);
                 d0 = {nh.x - cc * phi.y, nh.y + cc * phi.x};
             }
-            if constexpr (sizeof(T) == 4 && kWideUpdate) {
+            if constexpr (sizeof(T) == 4 && kWideUpdate && !NARROW) {
                 // FP32 state: the AB/BDF2 combination and the implicit diffusion in FP64, rounded once
                 const uint32_t key = step_key + (uint32_t)(r * g.nky + iky) * 0x9e3779b1u;
                 an = ab_bdf_update_wide(ist, plain, dt, diff, k2, B.a1[u], B.a2[u], d0, B.d1[u], B.d2[u], key);

@@ -95,6 +95,7 @@ _SIGS = {
     "shrt_get_variants": ([_VP, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                            ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "shrt_set_fusion": ([_VP, ctypes.c_int], ctypes.c_int),
+    "shrt_set_update_precision": ([_VP, ctypes.c_int], ctypes.c_int),
     "shrt_ensemble_create": ([ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int,
                               ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
     "shrt_ensemble_destroy": ([_VP], ctypes.c_int),
@@ -326,6 +327,10 @@ class Shrt:
         v = [ctypes.c_int() for _ in range(4)]
         _check(lib().shrt_get_variants(self.h, *[ctypes.byref(x) for x in v]))
         return dict(p1=v[0].value, p2=v[1].value, p3=v[2].value, fuse=v[3].value)
+
+    def set_update_precision(self, wide):
+        """FP32 contexts: state update in FP64 with stochastic rounding (True, default) or FP32 arithmetic."""
+        _check(lib().shrt_set_update_precision(self.h, 1 if wide else 0))
 
     def set_fusion(self, on):
         """Run captured batches with P3 of step n and P1 of step n+1 fused (default off)."""

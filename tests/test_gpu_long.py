@@ -67,7 +67,10 @@ def test_headline_workload_1000_steps(gpu):
     """The benchmark's workload itself: 2048^2, sigma = 1, nu = kappa = 1e-4,
     dt = 1e-4, omega = 0 and a seeded random-phase rho on 2 <= |k| <= 32
     (amplitude 1e-4 |k|^-1.5); FP32 (the benchmark's precision) and FP64
-    contexts against one FP64 oracle run over 1000 steps, every 250 steps."""
+    contexts against one FP64 oracle run over 1000 steps, every 250 steps.
+    A third context runs FP32 with the state update in FP32 arithmetic (the
+    reference's FP32 build, shrt_set_update_precision(0)); its error is
+    reported beside the default FP64-evaluated, stochastically rounded update."""
     n, chunk, nchunk = 2048, 250, 4
     p = dict(Lx=2 * np.pi, Ly=2 * np.pi, nu=1e-4, kappa=1e-4, sigma=1.0, g=1.0, rho0=1.0, rho0_prime=1.0,
              delt=1e-4, cfl_num=0.35, noise=0, rho_eps1=1e-2)
@@ -77,7 +80,12 @@ def test_headline_workload_1000_steps(gpu):
     gs = {prec: shrt.Shrt(nx=n, ny=n, precision=prec, graph_steps=20, **p) for prec in (8, 4)}
     for g in gs.values():
         g.initialize_seeded(**ic)
-    _row("2048^2 x 1000: step | FP64 omega rho | FP32 omega rho | energy enstrophy rho_var (FP64 rel, FP32 rel)")
+    narrow = shrt.Shrt(nx=n, ny=n, precision=4, graph_steps=20, **p)
+    narrow.set_update_precision(False)
+    narrow.initialize_seeded(**ic)
+    worst_narrow = 0.0
+    _row("2048^2 x 1000: step | FP64 omega rho | FP32 omega rho | FP32-arith omega rho | energy enstrophy rho_var "
+         "(FP64 rel, FP32 rel)")
     worst = {8: 0.0, 4: 0.0}
     for c in range(nchunk):
         o.step(chunk)
@@ -88,15 +96,21 @@ def test_headline_workload_1000_steps(gpu):
             e = [rel_l2(g.spectral(f), o.buffer(r)) for f, r in ((shrt.OMEGA, "aomg"), (shrt.RHO, "arho"))]
             worst[prec] = max(worst[prec], *e)
             row.append(f"{e[0]:.2e} {e[1]:.2e}")
+        narrow.time_advance(chunk)
+        en = [rel_l2(narrow.spectral(f), o.buffer(r)) for f, r in ((shrt.OMEGA, "aomg"), (shrt.RHO, "arho"))]
+        worst_narrow = max(worst_narrow, *en)
+        row.append(f"FP32-arith {en[0]:.2e} {en[1]:.2e}")
         dd = {prec: g.diagnostics() for prec, g in gs.items()}
         row.append(" ".join(f"{k} {do[k]:.6e} ({abs(dd[8][k] - do[k]) / do[k]:.1e}, {abs(dd[4][k] - do[k]) / do[k]:.1e})"
                             for k in ("energy", "enstrophy", "rho_var")))
         _row(" | ".join(row))
         for g in gs.values():
             assert g.state()["istep"] == (c + 1) * chunk
-    _row(f"worst field error FP64 {worst[8]:.2e}, FP32 {worst[4]:.2e}")
+    _row(f"worst field error FP64 {worst[8]:.2e}, FP32 {worst[4]:.2e}, FP32 with the update in FP32 "
+         f"arithmetic {worst_narrow:.2e}")
     assert worst[8] < 1e-10
     assert worst[4] < 1e-5
-    for g in gs.values():
+    assert worst_narrow < 1e-4
+    for g in list(gs.values()) + [narrow]:
         g.close()
     o.close()

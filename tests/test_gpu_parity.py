@@ -212,6 +212,32 @@ def test_small_case_full_length(gpu, prec):
     o.close()
 
 
+def test_fp32_update_precision_forms(gpu):
+    """The small case (128^2, sigma = 0.5, nu = kappa = 1e-3, dt = 1e-3) over 1000
+    steps in FP32 with both forms of the state update against the FP64 oracle:
+    evaluated in FP64 and rounded stochastically (the default) and in FP32
+    arithmetic (the reference's FP32 build).  Both are reported; the default
+    holds the 1e-5 bound."""
+    n = 128
+    p = dict(BASE, sigma=0.5, nu=1e-3, kappa=1e-3, delt=1e-3)
+    ic = dict(seed=20240601, kmin=2.0, kmax=8.0, amp=1e-3, expo=-1.0)
+    o = oracle.Oracle(threads=8, nx=n, ny=n, **p)
+    o.set_spectral("rho", oracle.seeded_spectrum(n, n, 2 * np.pi, 2 * np.pi, **ic))
+    o.step(1000)
+    errs = {}
+    for wide in (True, False):
+        g = shrt.Shrt(nx=n, ny=n, precision=4, graph_steps=20, **p)
+        g.set_update_precision(wide)
+        g.initialize_seeded(**ic)
+        g.time_advance(1000)
+        errs[wide] = max(rel_l2(g.spectral(f), o.buffer(r)) for f, r in ((shrt.OMEGA, "aomg"), (shrt.RHO, "arho")))
+        g.close()
+    print(f"128^2 x 1000 FP32: FP64-evaluated update {errs[True]:.2e}, FP32 arithmetic {errs[False]:.2e}", flush=True)
+    assert errs[True] < 1e-5
+    assert errs[False] < 1e-4
+    assert errs[False] > errs[True]
+
+
 @pytest.mark.parametrize("prec", [4, 8])
 def test_large_grid_4096(gpu, prec):
     """4096^2 for three steps from a seeded state on 4 <= |k| <= 64 against the
