@@ -238,21 +238,36 @@ def run_ensemble(shrt, args):
 CLOSURE_DEFAULT = dict(n_fine=256, n_coarse=64, n_mid=128, delt=1e-3, interval=20, spinup=1000, snaps=40,
                        nu=1.25e-4, rho0_prime=-1.0, amp=0.5, kmin=1.0, kmax_ic=16.0, kmax_net=16, width=16, layers=4)
 
+# The learned-closure configuration at full size: 2048^2 fine runs (shear U(0.5, 1.5),
+# g = 1, nu = kappa = 1e-4, delt 1.25e-4 so that max|vx + sigma*y| stays under the
+# CFL number) filtered to 256^2, 64 trajectories x 4000 steps (1000 of spin-up, then
+# a snapshot every 20 steps), a 4-layer width-32 network on 32 modes per axis, 16
+# held-out seeds; plain 256^2, 512^2, 1024^2 and 2048^2 runs give the
+# error-versus-GPU-time curve the learned 256^2 run is placed on.
+CLOSURE_CONFIG5 = dict(n_fine=2048, n_coarse=256, n_mid=512, delt=1.25e-4, interval=20, spinup=1000, snaps=150,
+                       nu=1e-4, rho0_prime=-1.0, amp=2.0, kmin=1.0, kmax_ic=32.0, kmax_net=32, width=32, layers=4,
+                       cfl=0.5, trajectories=64, val_trajectories=4, test_seeds=list(range(900, 916)),
+                       eval_steps=4000, eval_grids=[512, 1024, 2048], host_path=False, iters=3000, batch=16,
+                       kcut=32, guard=0.1)
+
 
 def run_closure(shrt, args):
     """Learned closure versus plain solvers: data from fine runs, device
     training with early stopping, then held-out seeds measured against the
-    filtered fine run (enstrophy spectrum and rho relative L2)."""
+    filtered fine run (enstrophy spectrum and rho relative L2), with the GPU
+    time of every run and the plain method's time at the learned run's error."""
     cl = _load("closure_pipe", os.path.join(REPO, "shearing-rt_amd", "closure.py"))
-    kw = dict(CLOSURE_DEFAULT)
-    kw.update(json.loads(args.closure))
+    spec = json.loads(args.closure)
+    kw = dict(CLOSURE_CONFIG5 if spec.pop("config5", False) else CLOSURE_DEFAULT)
+    kw.update(spec)
     ntrain, nval = kw.pop("trajectories", 64), kw.pop("val_trajectories", 4)
     iters, lr, batch = kw.pop("iters", 2000), kw.pop("lr", 5e-4), kw.pop("batch", 16)
     tests, evsteps = kw.pop("test_seeds", [900, 901, 902, 903]), kw.pop("eval_steps", 1000)
+    grids, host_path = kw.pop("eval_grids", None), kw.pop("host_path", True)
     cfg = cl.Config(**kw)
     p = cl.Pipeline(shrt, cfg)
     t0 = time.perf_counter()
-    x, y = p.generate(range(100, 100 + ntrain))
+    x, y = p.generate(range(100, 100 + ntrain), log=log)
     val = p.generate(range(800, 800 + nval))
     t_data = time.perf_counter() - t0
     log(f"closure: {len(x)} training samples ({p.dropped} non-finite dropped, fine-run CFL halvings "
@@ -261,19 +276,34 @@ def run_closure(shrt, args):
     t0 = time.perf_counter()
     losses = p.fit(x, y, iters=iters, batch=batch, lr=lr, val=val)
     t_train = time.perf_counter() - t0
+    del x, y
     log(f"closure: trained {iters} iterations in {t_train:.1f} s, best validation R2 {p.val_best:.3f}")
-    res = p.evaluate(tests, evsteps)
+    res = p.evaluate(tests, evsteps, grids=grids, host_path=host_path, log=log)
+    runs = list(res[0]["wall_s"].keys())
     mean = lambda run, key: float(np.mean([r[run][key] for r in res]))
     wall = lambda run: float(np.mean([r["wall_s"][run] for r in res]))
     out = {"metric": "learned closure: enstrophy-spectrum / rho relative L2 error vs filtered fine run",
-           "config": dict(kw, trajectories=ntrain, iters=iters, eval_steps=evsteps, test_seeds=tests),
-           "samples": int(len(x)), "data_s": t_data, "train_s": t_train, "train_loss_last": float(losses[-1]),
+           "config": dict(kw, trajectories=ntrain, iters=iters, eval_steps=evsteps, test_seeds=tests,
+                          eval_grids=grids),
+           "samples": int(p.nsamples), "data_s": t_data, "train_s": t_train, "train_loss_last": float(losses[-1]),
            "val_r2_best": p.val_best, "val_curve": p.val_curve,
            "error": {run: {"enstrophy_spectrum": mean(run, "enstrophy_spectrum"), "rho": mean(run, "rho")}
-                     for run in ("plain", "closure", "closure_device", "mid")},
-           "wall_s_per_run": {run: wall(run) for run in ("plain", "closure", "closure_device", "mid")},
-           "device_vs_host_omega_rel_l2": float(max(r["device_vs_host"] for r in res)),
-           "skipped_corrections": int(sum(r["skipped"] for r in res)), "per_seed": res}
+                     for run in runs},
+           "wall_s_per_run": {run: wall(run) for run in runs},
+           "skipped_corrections": int(sum(r.get("skipped_device", 0) for r in res)), "per_seed": res}
+    if host_path:
+        out["device_vs_host_omega_rel_l2"] = float(max(r["device_vs_host"] for r in res))
+    # time to error: the plain method's GPU time at the learned run's error, per measure
+    plain_runs = [r for r in runs if r == "plain" or r.startswith("plain_") or r == "mid"]
+    tte = {}
+    for key in ("enstrophy_spectrum", "rho"):
+        pts = [(wall(r), mean(r, key)) for r in plain_runs]
+        t_eq, how = cl.time_at_error(pts, mean("closure_device", key))
+        tte[key] = {"learned_s": wall("closure_device"), "learned_error": mean("closure_device", key),
+                    "plain_s_at_equal_error": t_eq, "bracket": how,
+                    "ratio_plain_over_learned": t_eq / wall("closure_device"),
+                    "plain_points": [{"run": r, "s": wall(r), "error": mean(r, key)} for r in plain_runs]}
+    out["time_to_error"] = tte
     p.net.close()
     print(json.dumps(out), flush=True)
 

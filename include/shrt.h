@@ -137,6 +137,14 @@ shrt_status shrt_synchronize(shrt_ctx* ctx);
 shrt_status shrt_get_spectral(shrt_ctx* ctx, int field, void* host);
 shrt_status shrt_set_spectral(shrt_ctx* ctx, int field, const void* host);
 
+/* The current spectral field restricted to the retained modes of a coarser
+ * nx_c x ny_c grid (new; the spectral filter of the learned closure's training
+ * data): host buffer of nkx_c*nky_c complex in the coarse grid's layout, row i
+ * from this grid's row i (kx >= 0) or nkx - (nkx_c - i) (kx < 0), its first
+ * nky_c modes; the same physical wavenumbers when Lx and Ly are equal.  Only
+ * those modes cross to the host. */
+shrt_status shrt_get_spectral_modes(shrt_ctx* ctx, int field, int nx_c, int ny_c, void* host);
+
 /* The spectral fields as update_shear hands them to k_data_bef at a remap step
  * (shear.cu:236, before the shearing_field calls of :238-268): right after a
  * step that remapped, omega and rho of the new state and the aphi of that
@@ -323,14 +331,16 @@ shrt_status shrt_closure_train(shrt_closure* net, int iters, int batch, float lr
 /* In-place correction of a solver context by the network, all on the device
  * (new; no reference counterpart).  Inputs: ktox of omega, rho and the stored
  * phi of ctx (sheared frame), each divided by scale_in[i], and sigma as the
- * fourth channel; the outputs times scale_out[j] are transformed with xtok and
- * added, times dt_tot, to omega and rho (phi follows omega as in
- * shrt_set_spectral).  Fallback: nothing changes, and *applied = 0, when an
+ * fourth channel; the outputs times scale_out[j] are transformed with xtok,
+ * restricted to |kx|, ky <= kcut when kcut > 0 (the network's pointwise path
+ * makes every scale; the correction is kept to the resolved band it is
+ * trained for), and added, times dt_tot, to omega and rho (phi follows omega
+ * as in shrt_set_spectral).  Fallback: nothing changes, and *applied = 0, when an
  * output or a correction is non-finite or |correction| * dt_tot > guard *
  * |field| (L2 over the stored modes) for either field.  The network's grid must be ctx's (square).
  * Same arithmetic as doing these steps through the host API. */
 shrt_status shrt_closure_correct(shrt_closure* net, shrt_ctx* ctx, const double* scale_in, double sigma,
-                                 const double* scale_out, double dt_tot, double guard, int* applied);
+                                 const double* scale_out, double dt_tot, double guard, int kcut, int* applied);
 
 /* Sizes derived by init_fft (fft.cu:91-96): out[0..7] = nkx, nky, nkxh, nkxh2,
  * nkxpad, ncy, nkx*nky, nkxh*ny. */

@@ -89,30 +89,83 @@ def gelu_grad(x):
     return 0.5 * (1 + th) + 0.5 * x * (1 - th * th) * GELU_C * (1 + 3 * 0.044715 * x * x)
 
 
+def factors(n, kmax):
+    """The separable factors of the basis: Fy[n][2(kmax+1)] = cos, sin of
+    2 pi ky iy/n (ky = 0..kmax); Fxc[n][2(2kmax+1)] = s cos, s sin and
+    Fxs = -s sin, s cos of 2 pi kx ix/n (kx = -kmax..kmax), s = sqrt(2)/n.
+    E's column pair of mode (kx, ky) is sum over the two stages, with the
+    (0, 0) mode's cosine scaled by 1/sqrt(2)."""
+    i = np.arange(n)
+    ky = np.arange(kmax + 1)
+    kx = np.arange(-kmax, kmax + 1)
+    b = 2 * np.pi * ((np.outer(i, ky)) % n) / n
+    a = 2 * np.pi * ((np.outer(i, kx)) % n) / n
+    Fy = np.empty((n, 2 * (kmax + 1)))
+    Fy[:, 0::2], Fy[:, 1::2] = np.cos(b), np.sin(b)
+    s = np.sqrt(2.0) / n
+    Fxc = np.empty((n, 2 * (2 * kmax + 1)))
+    Fxs = np.empty_like(Fxc)
+    Fxc[:, 0::2], Fxc[:, 1::2] = s * np.cos(a), s * np.sin(a)
+    Fxs[:, 0::2], Fxs[:, 1::2] = -s * np.sin(a), s * np.cos(a)
+    return Fy, Fxc, Fxs
+
+
 class ClosureOracle:
     """Forward and hand-written backward pass in FP64.  With bf16=True every
-    contraction rounds both operands to BF16 first, as the GPU's GEMMs do, so
-    that the two agree to FP32 accumulation order rather than to BF16 rounding."""
+    contraction rounds both operands to BF16 first, as the GPU's GEMMs do --
+    including the two stages (along y, then along x) in which the GPU applies
+    the basis -- so that the two agree to FP32 accumulation order rather than
+    to BF16 rounding."""
 
     def __init__(self, n, kmax, width, layers, bf16=False):
         self.n, self.kmax, self.C, self.L = n, kmax, width, layers
         self.E = basis(n, kmax)
         self.M = self.E.shape[1] // 2
         self.lay = Layout(width, layers, self.M)
+        self.bf = bf16
         self.q = globals()["bf16"] if bf16 else (lambda v: v)
-        self.Eq = self.q(self.E)
+        Fy, Fxc, Fxs = factors(n, kmax)
+        self.Fy, self.Fxc, self.Fxs = self.q(Fy), self.q(Fxc), self.q(Fxs)
+
+    def analysis(self, h):
+        """X = h E for h [..., n*n]; in BF16 mode as the GPU's two GEMM stages."""
+        if not self.bf:
+            return h @ self.E
+        q, n, K1 = self.q, self.n, self.kmax + 1
+        T = q(h).reshape(h.shape[:-1] + (n, n)) @ self.Fy            # [..., ix, (ky, c)]
+        Tc, Ts = q(T[..., 0::2]), q(T[..., 1::2])                     # [..., ix, ky]
+        O = np.swapaxes(Tc, -1, -2) @ self.Fxc + np.swapaxes(Ts, -1, -2) @ self.Fxs   # [..., ky, J]
+        X = O.reshape(h.shape[:-1] + (-1,))[..., 2 * self.kmax:].copy()
+        X[..., 0] *= np.sqrt(0.5)
+        return X
+
+    def synthesis(self, Y):
+        """Y E^T for coefficient rows Y [..., 2M]; in BF16 mode as the GPU's two stages."""
+        if not self.bf:
+            return Y @ self.E.T
+        q, n, K1 = self.q, self.n, self.kmax + 1
+        J = 2 * (2 * self.kmax + 1)
+        Yf = np.zeros(Y.shape[:-1] + (K1 * J,))
+        Yf[..., 2 * self.kmax:] = Y
+        Yf[..., 2 * self.kmax] *= np.sqrt(0.5)
+        Yr = q(Yf).reshape(Y.shape[:-1] + (K1, J))
+        U = np.empty(Y.shape[:-1] + (n, 2 * K1))
+        U[..., 0::2] = np.swapaxes(Yr @ self.Fxc.T, -1, -2)          # [..., ix, ky]
+        U[..., 1::2] = np.swapaxes(Yr @ self.Fxs.T, -1, -2)
+        z = q(U) @ self.Fy.T                                          # [..., ix, iy]
+        return z.reshape(Y.shape[:-1] + (n * n,))
 
     def spectral(self, h, a, b):
         """K h for h [B][C][NP]: per mode Yc = Xc a^T - Xs b^T, Ys = Xc b^T + Xs a^T."""
         q = self.q
-        X = q(h) @ self.Eq                              # [B][C][2M]
+        X = self.analysis(h)                            # [B][C][2M]
         Xc, Xs = q(X[..., 0::2]), q(X[..., 1::2])       # [B][C][M]
         a, b = q(a), q(b)
         Yc = np.einsum("moi,bim->bom", a, Xc) - np.einsum("moi,bim->bom", b, Xs)
         Ys = np.einsum("moi,bim->bom", b, Xc) + np.einsum("moi,bim->bom", a, Xs)
         Y = np.empty_like(X)
         Y[..., 0::2], Y[..., 1::2] = Yc, Ys
-        return q(Y) @ self.Eq.T, X
+        return self.synthesis(Y), X
 
     def forward(self, theta, x, keep=False):
         q = self.q
@@ -150,7 +203,7 @@ class ClosureOracle:
             g[lay.w[l]:lay.w[l] + C] = dz.sum(axis=(0, 2))
             g[lay.W[l]:lay.W[l] + C * C] = np.einsum("bop,bip->oi", q(dz), q(hs[l])).ravel()
             dh = np.einsum("oi,bop->bip", q(d["W"][l]), q(dz))
-            dY = q(dz) @ self.Eq
+            dY = self.analysis(dz)
             dYc, dYs = q(dY[..., 0::2]), q(dY[..., 1::2])
             X = Xs[l]
             Xc, Xsn = q(X[..., 0::2]), q(X[..., 1::2])
@@ -163,7 +216,7 @@ class ClosureOracle:
             g[lay.b[l]:lay.b[l] + M * C * C] = gb.ravel()
             dX = np.empty_like(X)
             dX[..., 0::2], dX[..., 1::2] = dXc, dXs
-            dh = dh + q(dX) @ self.Eq.T
+            dh = dh + self.synthesis(dX)
         g[lay.P:lay.P + 4 * C] = np.einsum("bcp,bip->ci", q(dh), q(x)).ravel()
         g[lay.p:lay.p + C] = dh.sum(axis=(0, 2))
         return loss, g

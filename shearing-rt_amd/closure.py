@@ -47,6 +47,7 @@ class Config:
         self.sigma_range = (0.5, 1.5)
         self.kmax_net, self.width, self.layers = 10, 32, 4
         self.guard = 0.5
+        self.kcut = 0                # > 0: corrections kept to |kx|, ky <= kcut
         for k, v in kw.items():
             if not hasattr(self, k):
                 raise KeyError(k)
@@ -114,12 +115,15 @@ class Pipeline:
         if cfg.spinup:
             f.time_advance(cfg.spinup)
         out = []
+        nc = cfg.n_coarse
         for j in range(count):
             if j:
                 f.time_advance(cfg.interval)
             st = f.state()
             sh, _ = f.shift()
-            out.append((self.filt(f.spectral(S.OMEGA)), self.filt(f.spectral(S.RHO)), st["time"],
+            # the filter runs on the device: only the coarse modes cross to the host
+            out.append((f.spectral_modes(S.OMEGA, nc).astype(np.complex128),
+                        f.spectral_modes(S.RHO, nc).astype(np.complex128), st["time"],
                         np.ascontiguousarray(sh[self.filt.rows]), st["halvings"]))
         f.close()
         return out
@@ -139,13 +143,17 @@ class Pipeline:
         x[3] = sigma
         return x
 
-    def generate(self, seeds):
-        """Inputs [S][4][n^2] (unscaled) and targets [S][2][n^2]."""
+    def generate(self, seeds, log=None):
+        """Inputs [S][4][n^2] (unscaled) and targets [S][2][n^2], float32."""
         cfg, S = self.cfg, self.shrt
-        xs, ys = [], []
+        seeds = list(seeds)
+        npts = cfg.n_coarse ** 2
+        x = np.empty((len(seeds) * cfg.snaps, 4, npts), dtype=np.float32)
+        y = np.empty((len(seeds) * cfg.snaps, 2, npts), dtype=np.float32)
+        k = 0
         saved = self.scale_in
         self.scale_in = np.ones(3)
-        for seed in seeds:
+        for si, seed in enumerate(seeds):
             sigma = cfg.sigma_of(seed)
             snaps = self.fine_snapshots(seed, cfg.snaps + 1)
             self.max_halvings = max(getattr(self, "max_halvings", 0), max(sn[4] for sn in snaps))
@@ -153,24 +161,28 @@ class Pipeline:
             for j in range(cfg.snaps):
                 om, rh, t, sh, _ = snaps[j]
                 self.load_state(c, om, rh, t, sh)
-                x = self.inputs_of(c, sigma)
+                xj = self.inputs_of(c, sigma)
                 c.time_advance(cfg.interval)
                 dt_tot = cfg.interval * cfg.delt
                 d_om = (snaps[j + 1][0] - c.spectral(S.OMEGA)) / dt_tot
                 d_rh = (snaps[j + 1][1] - c.spectral(S.RHO)) / dt_tot
-                xs.append(x)
-                ys.append(np.stack([c.ktox(d_om).ravel(), c.ktox(d_rh).ravel()]))
+                x[k] = xj
+                y[k, 0] = c.ktox(d_om).ravel()
+                y[k, 1] = c.ktox(d_rh).ravel()
+                k += 1
             c.close()
+            if log is not None and (si % 8 == 7 or si == len(seeds) - 1):
+                log(f"closure data: {si + 1}/{len(seeds)} trajectories")
         self.scale_in = saved
-        x, y = np.array(xs), np.array(ys)
+        x, y = x[:k], y[:k]
         ok = np.isfinite(x).all(axis=(1, 2)) & np.isfinite(y).all(axis=(1, 2))
         self.dropped = int((~ok).sum())   # samples from runs that blew up
-        return x[ok], y[ok]
+        return (x, y) if self.dropped == 0 else (x[ok], y[ok])
 
     def normalise(self, x, y=None):
         xn = x.copy()
-        xn[:, :3] /= self.scale_in[None, :, None]
-        return xn if y is None else (xn, y / self.scale_out[None, :, None])
+        xn[:, :3] /= self.scale_in[None, :, None].astype(x.dtype)
+        return xn if y is None else (xn, y / self.scale_out[None, :, None].astype(y.dtype))
 
     def predict(self, xn):
         """Network outputs (normalised units) for normalised inputs, in batches."""
@@ -187,13 +199,14 @@ class Pipeline:
         validation set val = (xv, yv) training runs in chunks and keeps the
         parameters with the best mean validation R^2 (early stopping)."""
         cfg = self.cfg
+        self.nsamples = len(x)
         self.scale_in = np.maximum(x[:, :3].std(axis=(0, 2)), 1e-30)
         self.scale_out = np.maximum(y.std(axis=(0, 2)), 1e-30)
         xn, yn = self.normalise(x, y)
         if self.net is None:
             self.net = self.shrt.Closure(cfg.n_coarse, cfg.kmax_net, cfg.width, cfg.layers, max_batch=batch,
                                          seed=seed)
-        self.net.set_data(xn.astype(np.float32), yn.astype(np.float32))
+        self.net.set_data(xn.astype(np.float32, copy=False), yn.astype(np.float32, copy=False))
         if val is None:
             return self.net.train(iters, batch, lr, seed=seed)
         xv, yv = self.normalise(*val)
@@ -218,7 +231,13 @@ class Pipeline:
         if not np.all(np.isfinite(out)):
             return None
         n = self.cfg.n_coarse
-        return [c.xtok((out[i] * self.scale_out[i]).reshape(n, n)) for i in range(2)]
+        corr = [c.xtok((out[i] * self.scale_out[i]).reshape(n, n)) for i in range(2)]
+        if self.cfg.kcut > 0:
+            kx, ky = k_ints(self.sc)
+            keep = (np.abs(kx)[:, None] <= self.cfg.kcut) & (ky[None, :] <= self.cfg.kcut)
+            for F in corr:
+                F[~keep] = 0
+        return corr
 
     def run_coarse(self, om, rh, t, sh, sigma, steps, closure=False, n=None, device=False):
         """A coarse (or, with n, another plain) run from a filtered state; returns
@@ -244,12 +263,17 @@ class Pipeline:
             shn[rows] = sh
             self.load_state(c, Om, Rh, t, shn)
         skipped = 0
+        c.synchronize()
         t0 = time.perf_counter()
         done = 0
+        if not closure:                # plain: one call, captured batches back to back
+            c.time_advance(steps)
+            done = steps
         while done < steps:
             k = min(cfg.interval, steps - done)
             if closure and device and k == cfg.interval:
-                if not self.net.correct(c, self.scale_in, sigma, self.scale_out, cfg.interval * cfg.delt, cfg.guard):
+                if not self.net.correct(c, self.scale_in, sigma, self.scale_out, cfg.interval * cfg.delt, cfg.guard,
+                                        cfg.kcut):
                     skipped += 1
             elif closure and k == cfg.interval:
                 corr = self.correction(c, sigma)
@@ -281,9 +305,13 @@ class Pipeline:
         rho = float(np.linalg.norm(run[1] - ref[1]) / np.linalg.norm(ref[1]))
         return dict(enstrophy_spectrum=ens, rho=rho)
 
-    def evaluate(self, seeds, steps):
-        """Held-out seeds: plain coarse, corrected coarse and the finer plain run
-        against the filtered fine run after `steps` steps from the first snapshot."""
+    def evaluate(self, seeds, steps, grids=None, host_path=True, log=None):
+        """Held-out seeds: plain coarse, corrected coarse (device path, and the
+        host path when host_path) and plain runs on the finer grids against the
+        filtered fine run after `steps` steps from the first snapshot; per run
+        the error measures and the wall time of the run (GPU work and launches,
+        all from the same filtered state, same step size)."""
+        grids = [self.cfg.n_mid] if grids is None else list(grids)
         res = []
         for seed in seeds:
             sigma = self.cfg.sigma_of(seed)
@@ -291,12 +319,43 @@ class Pipeline:
             om, rh, t, sh, _ = snaps[0]
             ref = snaps[-1][:2]
             plain, w_plain, _ = self.run_coarse(om, rh, t, sh, sigma, steps)
-            corr, w_corr, skipped = self.run_coarse(om, rh, t, sh, sigma, steps, closure=True)
             dev, w_dev, skipped_dev = self.run_coarse(om, rh, t, sh, sigma, steps, closure=True, device=True)
-            mid, w_mid, _ = self.run_coarse(om, rh, t, sh, sigma, steps, n=self.cfg.n_mid)
-            res.append(dict(seed=seed, sigma=sigma, plain=self.errors(plain, ref), closure=self.errors(corr, ref),
-                            closure_device=self.errors(dev, ref), mid=self.errors(mid, ref), skipped=skipped,
-                            skipped_device=skipped_dev,
-                            device_vs_host=float(np.linalg.norm(dev[0] - corr[0]) / np.linalg.norm(corr[0])),
-                            wall_s=dict(plain=w_plain, closure=w_corr, closure_device=w_dev, mid=w_mid)))
+            r = dict(seed=seed, sigma=sigma, plain=self.errors(plain, ref), closure_device=self.errors(dev, ref),
+                     skipped_device=skipped_dev, wall_s=dict(plain=w_plain, closure_device=w_dev))
+            if host_path:
+                corr, w_corr, skipped = self.run_coarse(om, rh, t, sh, sigma, steps, closure=True)
+                r.update(closure=self.errors(corr, ref), skipped=skipped,
+                         device_vs_host=float(np.linalg.norm(dev[0] - corr[0]) / np.linalg.norm(corr[0])))
+                r["wall_s"]["closure"] = w_corr
+            for n in grids:
+                out, w, _ = self.run_coarse(om, rh, t, sh, sigma, steps, n=n)
+                key = "mid" if (n == self.cfg.n_mid and len(grids) == 1) else f"plain_{n}"
+                r[key] = self.errors(out, ref)
+                r["wall_s"][key] = w
+            res.append(r)
+            if log is not None:
+                log(f"closure eval seed {seed}: " + ", ".join(
+                    f"{k} {r[k]['enstrophy_spectrum']:.3f}/{r[k]['rho']:.3f} ({r['wall_s'][k] * 1e3:.1f} ms)"
+                    for k in r["wall_s"]))
         return res
+
+
+def time_at_error(points, err):
+    """GPU time the plain method needs for error `err`, from its (time, error)
+    points by log-log interpolation between the two grids that bracket it.
+    Returns (time, bracket): bracket is "interpolated", "below" (err under every
+    plain error: the finest plain time is a lower bound) or "above" (err over
+    every plain error: the coarsest plain time is an upper bound)."""
+    pts = sorted(points)                  # by time
+    ts = np.array([p[0] for p in pts])
+    es = np.array([p[1] for p in pts])
+    if err <= es.min():
+        return float(ts[np.argmin(es)]), "below"
+    if err >= es[0]:
+        return float(ts[0]), "above"
+    for i in range(len(pts) - 1):
+        e0, e1 = es[i], es[i + 1]
+        if (e0 - err) * (e1 - err) <= 0 and e0 != e1:
+            w = (np.log(err) - np.log(e0)) / (np.log(e1) - np.log(e0))
+            return float(np.exp(np.log(ts[i]) + w * (np.log(ts[i + 1]) - np.log(ts[i])))), "interpolated"
+    return float(ts[np.argmin(np.abs(es - err))]), "nearest"

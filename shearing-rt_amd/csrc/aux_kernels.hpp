@@ -298,6 +298,17 @@ __global__ void k_view_copy(Geo g, const cplx<T>* __restrict__ in, cplx<T>* __re
 }
 
 // aphi = omega / kperp2_shear for modes 1..M-1, aphi[0] = 0 (fourier.cu:233-240)
+// Modes with |kx| > kcut or ky > kcut (integer wavenumbers, storage order of
+// fourier.cu:124-125) set to zero.
+template <typename T>
+__global__ void k_lowpass(Geo g, cplx<T>* __restrict__ f, int kcut) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)g.nkx * g.nky) return;
+    const int row = (int)(t / g.nky), iky = (int)(t - (long long)row * g.nky);
+    const int kx = row < g.nkxh ? row : row - g.nkx;
+    if (kx > kcut || -kx > kcut || iky > kcut) f[t] = czero<T>();
+}
+
 template <typename T>
 __global__ void k_phi_from_omega(Geo g, const cplx<T>* __restrict__ om, cplx<T>* __restrict__ phi,
                                  const double* kx, const double* ky, const double* s) {

@@ -12,6 +12,15 @@
 // basis E), per mode Yc = Xc a^T - Xs b^T, Ys = Xc b^T + Xs a^T (a complex
 // weight a + ib per mode and channel pair, so K_l commutes with translations),
 // K_l h = Y E^T.  Every contraction is a BF16 MFMA GEMM with FP32 accumulation.
+//
+// E is never formed: its columns s*cos(a + b), s*sin(a + b), a = 2 pi kx ix/N,
+// b = 2 pi ky iy/N, factor into a y stage (the cos b, sin b of ky = 0..kmax)
+// and an x stage (s*cos a, s*sin a of kx = -kmax..kmax), so the analysis is
+// two truncated-DFT GEMMs of O(N^2 kmax) work per channel instead of one of
+// O(N^2 kmax^2), and the synthesis is their transpose.  Coefficient rows are
+// stored as [ky][kx = -kmax..kmax][cos, sin] (pitch DP = (kmax+1)(4 kmax+2)):
+// the modes in the order above start at offset OFF = 2 kmax, the ky = 0,
+// kx < 0 entries before it are not modes (zero where a synthesis reads them).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -59,7 +68,11 @@ struct shrt_closure {
     long long oP = 0, op = 0, oQ = 0, oq = 0;
     std::vector<long long> oa, ob, oW, ow;
     hipStream_t stream = nullptr;
-    float* E = nullptr;   // [NP][D]
+    int K1 = 0, J = 0, DP = 0, OFF = 0;   // ky count, kx-row length (cos, sin), coefficient pitch, mode offset
+    float* Fy = nullptr;    // [N][2*K1]: cos b, sin b
+    float* Fxc = nullptr;   // [N][J]: s cos a, s sin a
+    float* Fxs = nullptr;   // [N][J]: -s sin a, s cos a
+    float* T = nullptr;     // [maxB*C*N][2*K1]: the y stage of an analysis or the x stage of a synthesis
     float *prm = nullptr, *grd = nullptr, *am = nullptr, *av = nullptr;
     float *x = nullptr, *out = nullptr, *dout = nullptr, *tgt = nullptr;
     std::vector<float*> h, z, X;
@@ -135,8 +148,41 @@ void mode_list(int kmax, std::vector<int>& kx, std::vector<int>& ky) {
         }
 }
 
+// Analysis of R rows h[r][ix*N + iy] (row pitch NP) into coefficient rows
+// X[r] (pitch DP, modes from OFF): X = h E.
+hipError_t analysis(shrt_closure* n, const float* h, float* X, int R) {
+    const int N = n->N, K2 = 2 * n->K1;
+    // y stage: T[(r, ix)][(ky, c)] = sum_iy h[r][ix][iy] (cos b, sin b)
+    hipError_t e = gemm(n, h, 0, N, 1, n->Fy, 0, K2, 1, n->T, 0, K2, 1, R * N, K2, N, 1, 1.f, 0.f);
+    if (e != hipSuccess) return e;
+    // x stage per row r: X[r][ky][kx, cs] = sum_ix Tc Fxc + Ts Fxs
+    const long long tz = (long long)N * K2;
+    if ((e = gemm(n, n->T, tz, 2, K2, n->Fxc, 0, n->J, 1, X, n->DP, n->J, 1, n->K1, n->J, N, R, 1.f, 0.f))) return e;
+    if ((e = gemm(n, n->T + 1, tz, 2, K2, n->Fxs, 0, n->J, 1, X, n->DP, n->J, 1, n->K1, n->J, N, R, 1.f, 1.f))) return e;
+    hipLaunchKernelGGL(k_scale_first, dim3((R + 255) / 256), dim3(256), 0, n->stream, X + n->OFF, (long long)n->DP, R,
+                       (float)M_SQRT1_2);
+    return hipGetLastError();
+}
+
+// Synthesis of R coefficient rows Y[r] (pitch DP; the entries before OFF
+// zero) into z[r][ix*N + iy] = (Y E^T)[r] + beta z.  Scales Y's (0, 0)
+// cosine in place.
+hipError_t synthesis(shrt_closure* n, float* Y, float* z, int R, float beta) {
+    const int N = n->N, K2 = 2 * n->K1;
+    hipLaunchKernelGGL(k_scale_first, dim3((R + 255) / 256), dim3(256), 0, n->stream, Y + n->OFF, (long long)n->DP, R,
+                       (float)M_SQRT1_2);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    // x stage: T[(r, ix)][(ky, c)] = sum_{kx, cs} Y[r][ky][kx, cs] (Fxc, Fxs)[ix][kx, cs]
+    const long long tz = (long long)N * K2;
+    if ((e = gemm(n, Y, n->DP, n->J, 1, n->Fxc, 0, 1, n->J, n->T, tz, 2, K2, n->K1, N, n->J, R, 1.f, 0.f))) return e;
+    if ((e = gemm(n, Y, n->DP, n->J, 1, n->Fxs, 0, 1, n->J, n->T + 1, tz, 2, K2, n->K1, N, n->J, R, 1.f, 0.f))) return e;
+    // y stage: z[(r, ix)][iy] = sum_{ky, c} T (cos b, sin b)
+    return gemm(n, n->T, 0, K2, 1, n->Fy, 0, 1, K2, z, 0, N, 1, R * N, N, K2, 1, 1.f, beta);
+}
+
 hipError_t forward(shrt_closure* n, int B) {
-    const int C = n->C, D = n->D, Mm = n->Mm;
+    const int C = n->C, D = n->DP, Mm = n->Mm;   // D: the coefficient row pitch
     const long long NP = n->NP;
     const float* P = n->prm + n->oP;
     hipError_t e = gemm(n, P, 0, 4, 1, n->x, 4 * NP, NP, 1, n->h[0], C * NP, NP, 1, C, (int)NP, 4, B, 1.f, 0.f);
@@ -147,17 +193,18 @@ hipError_t forward(shrt_closure* n, int B) {
         const float* a = n->prm + n->oa[l];
         const float* b = n->prm + n->ob[l];
         // analysis X = h E
-        if ((e = gemm(n, n->h[l], 0, NP, 1, n->E, 0, D, 1, n->X[l], 0, D, 1, B * C, D, (int)NP, 1, 1.f, 0.f))) return e;
+        if ((e = analysis(n, n->h[l], n->X[l], B * C))) return e;
         // per mode: Yc = Xc a^T - Xs b^T, Ys = Xc b^T + Xs a^T
-        const float* Xc = n->X[l];
-        const float* Xs = n->X[l] + 1;
+        const float* Xc = n->X[l] + n->OFF;
+        const float* Xs = n->X[l] + n->OFF + 1;
+        float* Yc = n->Y + n->OFF;
         const long long xi = (long long)C * D;
-        if ((e = gemm(n, Xc, 2, xi, D, a, (long long)C * C, 1, C, n->Y, 2, xi, D, B, C, C, Mm, 1.f, 0.f))) return e;
-        if ((e = gemm(n, Xs, 2, xi, D, b, (long long)C * C, 1, C, n->Y, 2, xi, D, B, C, C, Mm, -1.f, 1.f))) return e;
-        if ((e = gemm(n, Xc, 2, xi, D, b, (long long)C * C, 1, C, n->Y + 1, 2, xi, D, B, C, C, Mm, 1.f, 0.f))) return e;
-        if ((e = gemm(n, Xs, 2, xi, D, a, (long long)C * C, 1, C, n->Y + 1, 2, xi, D, B, C, C, Mm, 1.f, 1.f))) return e;
+        if ((e = gemm(n, Xc, 2, xi, D, a, (long long)C * C, 1, C, Yc, 2, xi, D, B, C, C, Mm, 1.f, 0.f))) return e;
+        if ((e = gemm(n, Xs, 2, xi, D, b, (long long)C * C, 1, C, Yc, 2, xi, D, B, C, C, Mm, -1.f, 1.f))) return e;
+        if ((e = gemm(n, Xc, 2, xi, D, b, (long long)C * C, 1, C, Yc + 1, 2, xi, D, B, C, C, Mm, 1.f, 0.f))) return e;
+        if ((e = gemm(n, Xs, 2, xi, D, a, (long long)C * C, 1, C, Yc + 1, 2, xi, D, B, C, C, Mm, 1.f, 1.f))) return e;
         // synthesis z = Y E^T, then z += W h
-        if ((e = gemm(n, n->Y, 0, D, 1, n->E, 0, 1, D, n->z[l], 0, NP, 1, B * C, (int)NP, D, 1, 1.f, 0.f))) return e;
+        if ((e = synthesis(n, n->Y, n->z[l], B * C, 0.f))) return e;
         if ((e = gemm(n, n->prm + n->oW[l], 0, C, 1, n->h[l], C * NP, NP, 1, n->z[l], C * NP, NP, 1, C, (int)NP, C, B,
                       1.f, 1.f)))
             return e;
@@ -174,7 +221,7 @@ hipError_t forward(shrt_closure* n, int B) {
 
 // Gradient of the loss whose output gradient is in dout; parameters' gradients to grd.
 hipError_t backward(shrt_closure* n, int B) {
-    const int C = n->C, D = n->D, Mm = n->Mm;
+    const int C = n->C, D = n->DP, Mm = n->Mm;   // D: the coefficient row pitch
     const long long NP = n->NP;
     float* G = n->grd;
     hipError_t e;
@@ -202,18 +249,19 @@ hipError_t backward(shrt_closure* n, int B) {
                       0.f)))
             return e;
         // spectral part: dY = dz E
-        if ((e = gemm(n, n->dz, 0, NP, 1, n->E, 0, D, 1, n->dY, 0, D, 1, B * C, D, (int)NP, 1, 1.f, 0.f))) return e;
+        if ((e = analysis(n, n->dz, n->dY, B * C))) return e;
         const long long xi = (long long)C * D, cc = (long long)C * C;
-        const float* dYc = n->dY;
-        const float* dYs = n->dY + 1;
+        const float* dYc = n->dY + n->OFF;
+        const float* dYs = n->dY + n->OFF + 1;
+        float* dXc = n->dX + n->OFF;
         // dXc = dYc a + dYs b, dXs = -dYc b + dYs a
-        if ((e = gemm(n, dYc, 2, xi, D, a, cc, C, 1, n->dX, 2, xi, D, B, C, C, Mm, 1.f, 0.f))) return e;
-        if ((e = gemm(n, dYs, 2, xi, D, b, cc, C, 1, n->dX, 2, xi, D, B, C, C, Mm, 1.f, 1.f))) return e;
-        if ((e = gemm(n, dYc, 2, xi, D, b, cc, C, 1, n->dX + 1, 2, xi, D, B, C, C, Mm, -1.f, 0.f))) return e;
-        if ((e = gemm(n, dYs, 2, xi, D, a, cc, C, 1, n->dX + 1, 2, xi, D, B, C, C, Mm, 1.f, 1.f))) return e;
+        if ((e = gemm(n, dYc, 2, xi, D, a, cc, C, 1, dXc, 2, xi, D, B, C, C, Mm, 1.f, 0.f))) return e;
+        if ((e = gemm(n, dYs, 2, xi, D, b, cc, C, 1, dXc, 2, xi, D, B, C, C, Mm, 1.f, 1.f))) return e;
+        if ((e = gemm(n, dYc, 2, xi, D, b, cc, C, 1, dXc + 1, 2, xi, D, B, C, C, Mm, -1.f, 0.f))) return e;
+        if ((e = gemm(n, dYs, 2, xi, D, a, cc, C, 1, dXc + 1, 2, xi, D, B, C, C, Mm, 1.f, 1.f))) return e;
         // da = dYc^T Xc + dYs^T Xs, db = -dYc^T Xs + dYs^T Xc (sums over the batch)
-        const float* Xc = n->X[l];
-        const float* Xs = n->X[l] + 1;
+        const float* Xc = n->X[l] + n->OFF;
+        const float* Xs = n->X[l] + n->OFF + 1;
         float* ga = G + n->oa[l];
         float* gb = G + n->ob[l];
         if ((e = gemm(n, dYc, 2, D, xi, Xc, 2, xi, D, ga, cc, C, 1, C, C, B, Mm, 1.f, 0.f))) return e;
@@ -221,7 +269,7 @@ hipError_t backward(shrt_closure* n, int B) {
         if ((e = gemm(n, dYc, 2, D, xi, Xs, 2, xi, D, gb, cc, C, 1, C, C, B, Mm, -1.f, 0.f))) return e;
         if ((e = gemm(n, dYs, 2, D, xi, Xc, 2, xi, D, gb, cc, C, 1, C, C, B, Mm, 1.f, 1.f))) return e;
         // dh += dX E^T
-        if ((e = gemm(n, n->dX, 0, D, 1, n->E, 0, 1, D, n->dh, 0, NP, 1, B * C, (int)NP, D, 1, 1.f, 1.f))) return e;
+        if ((e = synthesis(n, n->dX, n->dh, B * C, 1.f))) return e;
     }
     // lift
     if ((e = gemm(n, n->dh, C * NP, NP, 1, n->x, 4 * NP, 1, NP, n->parts, (long long)C * 4, 4, 1, C, 4, (int)NP, B, 1.f,
@@ -305,12 +353,19 @@ shrt_status shrt_closure_create(int n_grid, int kmax, int width, int layers, int
     };
     if (hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking) != hipSuccess)
         return fail_free(cfail(SHRT_ERR_DEVICE, "closure: stream"));
-    const long long NP = n->NP, B = max_batch, D = n->D;
+    n->K1 = kmax + 1;
+    n->J = 2 * (2 * kmax + 1);
+    n->DP = n->K1 * n->J;
+    n->OFF = 2 * kmax;
+    const long long NP = n->NP, B = max_batch, D = n->DP;
     shrt_status st = SHRT_OK;
     auto A = [&](float** p, long long count) {
         if (st == SHRT_OK) st = nn_alloc(n, reinterpret_cast<void**>(p), sizeof(float) * (size_t)count);
     };
-    A(&n->E, NP * D);
+    A(&n->Fy, (long long)n_grid * 2 * n->K1);
+    A(&n->Fxc, (long long)n_grid * n->J);
+    A(&n->Fxs, (long long)n_grid * n->J);
+    A(&n->T, B * C * n_grid * 2 * n->K1);
     A(&n->prm, n->np); A(&n->grd, n->np); A(&n->am, n->np); A(&n->av, n->np);
     A(&n->x, B * 4 * NP); A(&n->out, B * 2 * NP); A(&n->dout, B * 2 * NP); A(&n->tgt, B * 2 * NP);
     n->h.assign(layers + 1, nullptr);
@@ -327,26 +382,33 @@ shrt_status shrt_closure_create(int n_grid, int kmax, int width, int layers, int
     if (st == SHRT_OK) st = nn_alloc(n, reinterpret_cast<void**>(&n->loss), sizeof(double));
     if (st != SHRT_OK) return fail_free(st);
 
-    // basis: column 2m = cos, 2m+1 = sin of mode m, orthonormal over the grid
+    // the factors of the orthonormal cos/sin basis (column 2m = cos, 2m+1 = sin
+    // of mode m; s = sqrt(2)/N, the (0, 0) mode's 1/N by k_scale_first)
     {
-        std::vector<double> ct(n_grid), stb(n_grid);
-        for (int j = 0; j < n_grid; ++j) {
-            ct[j] = std::cos(2.0 * M_PI * j / n_grid);
-            stb[j] = std::sin(2.0 * M_PI * j / n_grid);
-        }
-        std::vector<float> Eh((size_t)NP * D);
-        for (int ix = 0; ix < n_grid; ++ix)
-            for (int iy = 0; iy < n_grid; ++iy) {
-                float* row = Eh.data() + ((size_t)ix * n_grid + iy) * D;
-                for (int m = 0; m < n->Mm; ++m) {
-                    const int j = (int)((((long long)mkx[m] * ix + (long long)mky[m] * iy) % n_grid + n_grid) % n_grid);
-                    const bool zero = mkx[m] == 0 && mky[m] == 0;
-                    const double s = zero ? 1.0 / n_grid : std::sqrt(2.0) / n_grid;
-                    row[2 * m] = (float)(s * ct[j]);
-                    row[2 * m + 1] = zero ? 0.f : (float)(s * stb[j]);
-                }
+        const int N = n_grid, K1 = n->K1, Jx = n->J;
+        std::vector<float> fy((size_t)N * 2 * K1), fxc((size_t)N * Jx), fxs((size_t)N * Jx);
+        auto ang = [&](long long k, int i) { return 2.0 * M_PI * (double)(((k * i) % N + N) % N) / N; };
+        const double sc = std::sqrt(2.0) / N;
+        for (int i = 0; i < N; ++i) {
+            for (int ky = 0; ky < K1; ++ky) {
+                fy[(size_t)i * 2 * K1 + 2 * ky] = (float)std::cos(ang(ky, i));
+                fy[(size_t)i * 2 * K1 + 2 * ky + 1] = (float)std::sin(ang(ky, i));
             }
-        NN_TRY(hipMemcpy(n->E, Eh.data(), sizeof(float) * Eh.size(), hipMemcpyHostToDevice));
+            for (int kx = -kmax; kx <= kmax; ++kx) {
+                const double c = std::cos(ang(kx, i)), sn = std::sin(ang(kx, i));
+                const size_t o = (size_t)i * Jx + 2 * (kx + kmax);
+                fxc[o] = (float)(sc * c);
+                fxc[o + 1] = (float)(sc * sn);
+                fxs[o] = (float)(-sc * sn);
+                fxs[o + 1] = (float)(sc * c);
+            }
+        }
+        NN_TRY(hipMemcpy(n->Fy, fy.data(), sizeof(float) * fy.size(), hipMemcpyHostToDevice));
+        NN_TRY(hipMemcpy(n->Fxc, fxc.data(), sizeof(float) * fxc.size(), hipMemcpyHostToDevice));
+        NN_TRY(hipMemcpy(n->Fxs, fxs.data(), sizeof(float) * fxs.size(), hipMemcpyHostToDevice));
+        // the non-mode entries of the synthesis inputs stay zero
+        NN_TRY(hipMemset(n->Y, 0, sizeof(float) * B * C * D));
+        NN_TRY(hipMemset(n->dX, 0, sizeof(float) * B * C * D));
     }
     // parameters: dense weights U(-1/sqrt(fan_in), 1/sqrt(fan_in)), spectral
     // weights U(0, 1)/(C*C), biases 0

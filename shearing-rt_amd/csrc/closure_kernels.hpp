@@ -148,6 +148,14 @@ __global__ void k_bias_act(float* z, float* h, const float* bias, int C, long lo
 }
 
 // dz = dh * gelu'(z)
+// The (0, 0) mode's cosine coefficient of `rows` coefficient rows scaled by s
+// (the separable transforms carry sqrt(2)/N for every mode; that mode's basis
+// column is 1/N).
+__global__ void k_scale_first(float* base, long long pitch, int rows, float s) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < rows) base[(long long)r * pitch] *= s;
+}
+
 __global__ void k_gelu_back(const float* dh, const float* z, float* dz, long long total) {
     for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total; e += (long long)gridDim.x * blockDim.x)
         dz[e] = dh[e] * gelu_grad(z[e]);

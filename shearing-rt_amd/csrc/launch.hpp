@@ -42,6 +42,7 @@ struct Launch {
     static hipError_t phi_from_omega(const Geo& g, const void* om, void* phi, const double* kx,
                                      const double* ky, const double* sh, hipStream_t s);
     static hipError_t fill_zero(void* p, long long n, hipStream_t s);
+    static hipError_t lowpass(const Geo& g, void* spec, int kcut, hipStream_t s);
     static hipError_t mul_acc(const void* a, const void* b, void* out, long long n, int negate, int accumulate,
                               hipStream_t s);
     static hipError_t copy16(const void* src, void* dst, long long nwords, hipStream_t s);

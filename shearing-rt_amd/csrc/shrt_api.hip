@@ -1007,6 +1007,44 @@ shrt_status shrt_get_spectral(shrt_ctx* c, int field, void* host) {
     return SHRT_OK;
 }
 
+shrt_status shrt_get_spectral_modes(shrt_ctx* c, int field, int nx_c, int ny_c, void* host) {
+    if (!c || !host) return fail(SHRT_ERR_INVALID, "null argument");
+    int sz[8];
+    if (shrt_sizes(nx_c, ny_c, sz) != SHRT_OK) return fail(SHRT_ERR_INVALID, "bad coarse grid");
+    const int nkx_c = sz[0], nky_c = sz[1], nkxh_c = sz[2];
+    if (nkxh_c > c->g.nkxh || nky_c > c->g.nky)
+        return fail(SHRT_ERR_INVALID, "shrt_get_spectral_modes: the coarse grid keeps modes this grid does not");
+    StepCtl v;
+    shrt_status st = read_ctl(c, &v);
+    if (st != SHRT_OK) return st;
+    const int slot = slot_a1(v.istep);
+    const void* src;
+    int ep;
+    switch (field) {
+        case SHRT_OMEGA: src = c->ring[0][slot]; ep = v.slot_epoch_a[slot]; break;
+        case SHRT_RHO: src = c->ring[2][slot]; ep = v.slot_epoch_a[slot]; break;
+        case SHRT_PHI: src = c->aphi; ep = v.phi_epoch; break;
+        default: return fail(SHRT_ERR_INVALID, "unknown spectral field");
+    }
+    const int* jmp;
+    if ((st = view_for(c, v, ep, &jmp)) != SHRT_OK) return st;
+    if (jmp) {
+        HIP_TRY(view_copy(c, src, c->tmp_spec, jmp));
+        src = c->tmp_spec;
+    }
+    // coarse row i is fine row i (kx >= 0) or nkx - (nkx_c - i) (kx < 0); its first nky_c modes
+    const size_t es = c->esz, pitch_f = (size_t)c->g.nky * es, pitch_c = (size_t)nky_c * es;
+    const char* s8 = static_cast<const char*>(src);
+    char* h8 = static_cast<char*>(host);
+    HIP_TRY(hipMemcpy2DAsync(h8, pitch_c, s8, pitch_f, pitch_c, nkxh_c, hipMemcpyDeviceToHost, c->stream));
+    if (nkx_c > nkxh_c)
+        HIP_TRY(hipMemcpy2DAsync(h8 + (size_t)nkxh_c * pitch_c, pitch_c,
+                                 s8 + (size_t)(c->g.nkx - (nkx_c - nkxh_c)) * pitch_f, pitch_f, pitch_c,
+                                 nkx_c - nkxh_c, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return SHRT_OK;
+}
+
 shrt_status shrt_get_spectral_unremapped(shrt_ctx* c, int field, void* host) {
     if (!c || !host) return fail(SHRT_ERR_INVALID, "null argument");
     StepCtl v;
@@ -1532,7 +1570,7 @@ namespace {
 
 template <typename T>
 shrt_status closure_correct_t(shrt_closure* net, shrt_ctx* c, const double* scale_in, double sigma,
-                              const double* scale_out, double dt_tot, double guard, int* applied) {
+                              const double* scale_out, double dt_tot, double guard, int kcut, int* applied) {
     StepCtl v;
     shrt_status st = read_ctl(c, &v);
     if (st != SHRT_OK) return st;
@@ -1558,6 +1596,7 @@ shrt_status closure_correct_t(shrt_closure* net, shrt_ctx* c, const double* scal
     for (int i = 0; i < 2; ++i) {
         HIP_TRY(Launch<T>::net_out(out + i * NP, c->tmp_phys, NP, scale_out[i], c->stream));
         HIP_TRY(xtok_dev(c, c->tmp_phys, corr[i]));
+        if (kcut > 0) HIP_TRY(Launch<T>::lowpass(c->g, corr[i], kcut, c->stream));
     }
     // the state in the current frame, norms and the finiteness of the output
     void* cur[2] = {c->tmp_spec, c->tmp_spec2};
@@ -1595,7 +1634,7 @@ shrt_status closure_correct_t(shrt_closure* net, shrt_ctx* c, const double* scal
 extern "C" {
 
 shrt_status shrt_closure_correct(shrt_closure* net, shrt_ctx* c, const double* scale_in, double sigma,
-                                 const double* scale_out, double dt_tot, double guard, int* applied) {
+                                 const double* scale_out, double dt_tot, double guard, int kcut, int* applied) {
     if (!net || !c || !scale_in || !scale_out || !applied) return fail(SHRT_ERR_INVALID, "null argument");
     *applied = 0;
     if (closure_grid(net) != c->g.nx || c->g.nx != c->g.ny)
@@ -1606,8 +1645,8 @@ shrt_status shrt_closure_correct(shrt_closure* net, shrt_ctx* c, const double* s
         return fail(SHRT_ERR_INVALID, "non-finite dt, guard or sigma");
     if (2 * (size_t)c->g.nkx * c->g.nky > 5 * (size_t)c->g.I)
         return fail(SHRT_ERR_UNSUPPORTED, "correction scratch too small for this grid");
-    return c->prec == 4 ? closure_correct_t<float>(net, c, scale_in, sigma, scale_out, dt_tot, guard, applied)
-                        : closure_correct_t<double>(net, c, scale_in, sigma, scale_out, dt_tot, guard, applied);
+    return c->prec == 4 ? closure_correct_t<float>(net, c, scale_in, sigma, scale_out, dt_tot, guard, kcut, applied)
+                        : closure_correct_t<double>(net, c, scale_in, sigma, scale_out, dt_tot, guard, kcut, applied);
 }
 
 }  // extern "C"

@@ -241,6 +241,14 @@ hipError_t Launch<R>::phi_from_omega(const Geo& g, const void* om, void* phi, co
 }
 
 template <>
+hipError_t Launch<R>::lowpass(const Geo& g, void* spec, int kcut, hipStream_t s) {
+    const long long M = (long long)g.nkx * g.nky;
+    hipLaunchKernelGGL((k_lowpass<R>), dim3((unsigned)((M + 255) / 256)), dim3(256), 0, s, g, static_cast<cplx<R>*>(spec),
+                       kcut);
+    return hipGetLastError();
+}
+
+template <>
 hipError_t Launch<R>::fill_zero(void* p, long long n, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL((k_fill_zero<R>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,

@@ -83,6 +83,7 @@ _SIGS = {
     "shrt_get_spectral": ([_VP, ctypes.c_int, _VP], ctypes.c_int),
     "shrt_set_spectral": ([_VP, ctypes.c_int, _VP], ctypes.c_int),
     "shrt_get_spectral_unremapped": ([_VP, ctypes.c_int, _VP], ctypes.c_int),
+    "shrt_get_spectral_modes": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP], ctypes.c_int),
     "shrt_xtok_noalloc": ([_VP, _VP, _VP], ctypes.c_int),
     "shrt_get_buffer": ([_VP, ctypes.c_int, ctypes.c_int, _VP], ctypes.c_int),
     "shrt_set_buffer": ([_VP, ctypes.c_int, ctypes.c_int, _VP], ctypes.c_int),
@@ -113,7 +114,7 @@ _SIGS = {
     "shrt_closure_set_data": ([_VP, _VP, _VP, ctypes.c_int], ctypes.c_int),
     "shrt_closure_train": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint64, _VP], ctypes.c_int),
     "shrt_closure_correct": ([_VP, _VP, _VP, ctypes.c_double, _VP, ctypes.c_double, ctypes.c_double,
-                              ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+                              ctypes.c_int, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "shrt_shearing_field": ([_VP, _VP, _VP, _VP], ctypes.c_int),
     "shrt_get_time": ([_VP, ctypes.POINTER(Time)], ctypes.c_int),
     "shrt_set_time": ([_VP, ctypes.c_int64, ctypes.c_double, ctypes.c_double], ctypes.c_int),
@@ -227,6 +228,13 @@ class Shrt:
     def spectral(self, field):
         out = np.zeros(self.shape_k, dtype=self.cplx)
         _check(lib().shrt_get_spectral(self.h, field, _ptr(out)))
+        return out
+
+    def spectral_modes(self, field, nx_c, ny_c=None):
+        """The field on the retained modes of a coarser grid (a spectral filter on the device)."""
+        sc = sizes(nx_c, ny_c or nx_c)
+        out = np.zeros((sc["nkx"], sc["nky"]), dtype=self.cplx)
+        _check(lib().shrt_get_spectral_modes(self.h, field, nx_c, ny_c or nx_c, _ptr(out)))
         return out
 
     def spectral_unremapped(self, field):
@@ -428,7 +436,7 @@ class Closure:
         _check(lib().shrt_closure_train(self.h, int(iters), int(batch), float(lr), int(seed), _ptr(losses)))
         return losses
 
-    def correct(self, ctx, scale_in, sigma, scale_out, dt_tot, guard):
+    def correct(self, ctx, scale_in, sigma, scale_out, dt_tot, guard, kcut=0):
         """Correct the solver context `ctx` (a Shrt) in place on the device;
         returns False when the fallback left it unchanged."""
         si = np.ascontiguousarray(scale_in, dtype=np.float64)
@@ -436,7 +444,7 @@ class Closure:
         assert si.shape == (3,) and so.shape == (2,)
         applied = ctypes.c_int()
         _check(lib().shrt_closure_correct(self.h, ctx.h, _ptr(si), float(sigma), _ptr(so), float(dt_tot),
-                                          float(guard), ctypes.byref(applied)))
+                                          float(guard), int(kcut), ctypes.byref(applied)))
         return bool(applied.value)
 
     def close(self):

@@ -81,6 +81,6 @@ def test_null_arguments():
     assert L.shrt_get_physical(None, 5, 0, None) == 1
     assert L.shrt_get_variants(None, None, None, None, None) == 1
     assert L.shrt_set_variants(None, 0, 0, 0) == 1
-    assert L.shrt_closure_correct(None, None, None, 1.0, None, 0.1, 1.0, None) == 1
+    assert L.shrt_closure_correct(None, None, None, 1.0, None, 0.1, 1.0, 0, None) == 1
     assert L.shrt_copy_bench(0, 1, None) == 1
     assert L.shrt_stream_copy_bench(24, 1, None) == 1

@@ -73,3 +73,26 @@ def test_translation_equivariance():
     y = o.forward(th, x.reshape(1, 4, n * n)).reshape(1, 2, n, n)
     ys = o.forward(th, sx.reshape(1, 4, n * n)).reshape(1, 2, n, n)
     assert np.abs(np.roll(y, (3, -5), axis=(2, 3)) - ys).max() < 1e-12
+
+
+@pytest.mark.parametrize("n,kmax", [(16, 5), (32, 10), (64, 8)])
+def test_separable_factors_reproduce_the_basis(n, kmax):
+    """The two-stage form the GPU applies (a y stage of cos/sin of ky, then an x
+    stage of cos/sin of kx, the (0, 0) cosine scaled by 1/sqrt(2)) equals the
+    dense orthonormal basis: analysis h E and synthesis Y E^T."""
+    E = co.basis(n, kmax)
+    exact = co.ClosureOracle(n, kmax, 2, 1)
+    rng = np.random.default_rng(n + kmax)
+    h = rng.standard_normal((3, n * n))
+    Fy, Fxc, Fxs = co.factors(n, kmax)
+    T = h.reshape(3, n, n) @ Fy
+    O = np.swapaxes(T[..., 0::2], -1, -2) @ Fxc + np.swapaxes(T[..., 1::2], -1, -2) @ Fxs
+    X = O.reshape(3, -1)[:, 2 * kmax:].copy()
+    X[:, 0] *= np.sqrt(0.5)
+    np.testing.assert_allclose(X, h @ E, atol=1e-12)
+    # the BF16 emulation of the GPU's stages stays within BF16 rounding of the exact transform
+    q = co.ClosureOracle(n, kmax, 2, 1, bf16=True)
+    assert np.linalg.norm(q.analysis(h) - h @ E) < 1e-2 * np.linalg.norm(h @ E)
+    Y = rng.standard_normal((3, E.shape[1]))
+    assert np.linalg.norm(q.synthesis(Y) - Y @ E.T) < 1e-2 * np.linalg.norm(Y @ E.T)
+    np.testing.assert_allclose(exact.synthesis(Y), Y @ E.T, atol=1e-12)

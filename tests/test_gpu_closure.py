@@ -158,18 +158,19 @@ def _pipeline_module():
     return cl
 
 
-@pytest.mark.parametrize("prec", [4, 8])
-def test_device_correction_matches_host(prec):
+@pytest.mark.parametrize("prec,kcut", [(4, 0), (8, 0), (4, 5)])
+def test_device_correction_matches_host(prec, kcut):
     """shrt_closure_correct against the same correction made through the host
     API (ktox of the stored fields, forward, xtok, set_spectral): the same
     state afterwards, bit for bit, also right after a remap (pending view) and
     over the steps that follow; the fallback (tiny guard, non-finite state)
-    leaves the state unchanged."""
+    leaves the state unchanged.  With kcut the corrections keep |kx|, ky <= kcut
+    on both paths."""
     cl = _pipeline_module()
     n = 32
     print(f"device closure correction prec={prec}", flush=True)
     cfg = cl.Config(n_coarse=n, n_fine=64, n_mid=64, delt=1e-2, interval=10, cfl=5.0, width=8, kmax_net=6,
-                    layers=2, rho0_prime=-1.0)
+                    layers=2, rho0_prime=-1.0, kcut=kcut)
     p = cl.Pipeline(shrt, cfg)
     p.net = shrt.Closure(n, 6, 8, 2, max_batch=1, seed=5)
     p.scale_in = np.array([0.7, 0.3, 0.05])
@@ -194,7 +195,7 @@ def test_device_correction_matches_host(prec):
         om, rh = a.spectral(shrt.OMEGA), a.spectral(shrt.RHO)
         a.set_spectral(shrt.OMEGA, om + dt * corr[0])
         a.set_spectral(shrt.RHO, rh + dt * corr[1])
-        assert p.net.correct(b, p.scale_in, sigma, p.scale_out, dt, 10.0)
+        assert p.net.correct(b, p.scale_in, sigma, p.scale_out, dt, 10.0, kcut)
         same(a, b)
         assert np.linalg.norm(a.spectral(shrt.OMEGA) - om) > 0
     assert b.state()["remap_count"] == 1
@@ -217,4 +218,43 @@ def test_device_correction_matches_host(prec):
         p.net.correct(b, np.zeros(3), sigma, p.scale_out, dt, 10.0)
     a.close()
     b.close()
+    p.net.close()
+
+
+def test_time_to_error_pipeline(gpu):
+    """The measurement of the full configuration (bench.py --closure with
+    config5) at reduced size: 256^2 fine runs filtered on the device to 64^2,
+    plain 64^2, 128^2 and 256^2 runs and the learned 64^2 run from the same
+    filtered states on two held-out seeds, each timed, and the plain method's
+    time at the learned run's error.  Checks the measurement itself: finite
+    errors, the finest plain grid closest to the filtered fine run, and a
+    time ratio bracketed by the plain curve."""
+    import importlib.util
+    here = os.path.dirname(os.path.abspath(__file__))
+    spec = importlib.util.spec_from_file_location("closure_pipe", os.path.join(here, "..", "shearing-rt_amd",
+                                                                               "closure.py"))
+    cl = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(cl)
+    print("time-to-error pipeline 256^2 -> 64^2", flush=True)
+    cfg = cl.Config(n_fine=256, n_coarse=64, n_mid=128, delt=5e-4, interval=20, spinup=400, snaps=20, nu=1e-4,
+                    rho0_prime=-1.0, amp=1.0, kmin=1.0, kmax_ic=16.0, kmax_net=8, width=16, layers=2, cfl=0.5,
+                    kcut=8, guard=0.1)
+    p = cl.Pipeline(shrt, cfg)
+    x, y = p.generate(range(100, 108))
+    p.fit(x, y, iters=200, batch=8, lr=5e-4)
+    res = p.evaluate([900, 901], 400, grids=[128, 256], host_path=False)
+    runs = ["plain", "plain_128", "plain_256", "closure_device"]
+    for r in res:
+        for k in runs:
+            assert np.isfinite(r[k]["enstrophy_spectrum"]) and np.isfinite(r[k]["rho"]) and r["wall_s"][k] > 0, k
+        assert r["plain_256"]["enstrophy_spectrum"] < r["plain"]["enstrophy_spectrum"]
+        print("  seed", r["seed"], {k: round(r[k]["enstrophy_spectrum"], 4) for k in runs},
+              {k: round(r["wall_s"][k] * 1e3, 2) for k in runs})
+    mean = lambda k: float(np.mean([r[k]["enstrophy_spectrum"] for r in res]))
+    wall = lambda k: float(np.mean([r["wall_s"][k] for r in res]))
+    pts = [(wall(k), mean(k)) for k in runs[:3]]
+    t_eq, how = cl.time_at_error(pts, mean("closure_device"))
+    assert how in ("interpolated", "below", "above", "nearest")
+    assert min(w for w, _ in pts) <= t_eq <= max(w for w, _ in pts)
+    print(f"  plain time at the learned error: {t_eq * 1e3:.2f} ms ({how}); learned {wall('closure_device') * 1e3:.2f} ms")
     p.net.close()

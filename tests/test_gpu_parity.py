@@ -393,6 +393,26 @@ def test_spectral_before_remap(gpu, prec):
 
 
 @pytest.mark.parametrize("prec", [8, 4])
+def test_spectral_modes_filter(gpu, prec):
+    """shrt_get_spectral_modes equals cutting the coarse grid's retained modes out
+    of the full spectrum on the host (rows kx >= 0 and kx < 0 of the coarse grid,
+    its first nky modes), bit for bit, also while a remap is pending."""
+    import importlib.util, os
+    from conftest import REPO
+    spec = importlib.util.spec_from_file_location("closure_pipe", os.path.join(REPO, "shearing-rt_amd", "closure.py"))
+    cl = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(cl)
+    o, g = _remapped_pair(prec)   # right after a remap: the view applies
+    for nc in (8, 16, 32):
+        filt = cl.Filter(shrt.sizes(32, 32), shrt.sizes(nc, nc))
+        for f in (shrt.OMEGA, shrt.RHO, shrt.PHI):
+            got = g.spectral_modes(f, nc)
+            assert np.array_equal(got.view(np.uint8), filt(g.spectral(f)).view(np.uint8)), (nc, f)
+    with pytest.raises(RuntimeError):
+        g.spectral_modes(shrt.OMEGA, 64)
+
+
+@pytest.mark.parametrize("prec", [8, 4])
 @pytest.mark.parametrize("new_istep", [100, 101, 105, 2])
 def test_set_time_keeps_state(gpu, prec, new_istep):
     """shrt_set_time right after a remap, with a step count in each class mod 3 (the
