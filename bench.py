@@ -57,15 +57,20 @@ def kernel_bytes(s, c):
 
 
 def read_traffic(name):
-    """HBM bytes per launch measured with rocprofv3 --pmc, if recorded."""
+    """HBM bytes per launch of a step kernel as two rocprofv3 --pmc runs of this
+    bench measured them (FETCH_SIZE, WRITE_SIZE, calibrated by a copy of known
+    size; profiles/traffic.json.txt, written by --pmc-summary), with the
+    profiled kernel's name: (bytes, source) or (None, None) if not recorded."""
     path = os.path.join(REPO, "profiles", "traffic.json.txt")
     if not os.path.exists(path):
-        return None
+        return None, None
     try:
         d = json.load(open(path)).get(name)
-        return None if d is None else d["bytes"]
+        if d is None:
+            return None, None
+        return d["bytes"], f"profiles/traffic.json.txt: rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE of {d.get('kernel', name)}"
     except Exception:
-        return None
+        return None, None
 
 
 def pmc_summary(fetch_csv, write_csv, cal_fetch_csv, cal_write_csv, cal_bytes):
@@ -102,7 +107,7 @@ def pmc_summary(fetch_csv, write_csv, cal_fetch_csv, cal_write_csv, cal_bytes):
             continue
         key = f"{m.group(1)}_{m.group(3)}_{'fp32' if m.group(2) == 'float' else 'fp64'}"
         out[key] = {"bytes": f_cal * fv * 1024.0 + w_cal * write[k] * 1024.0,
-                    "read_bytes": f_cal * fv * 1024.0, "write_bytes": w_cal * write[k] * 1024.0}
+                    "read_bytes": f_cal * fv * 1024.0, "write_bytes": w_cal * write[k] * 1024.0, "kernel": k}
     return out
 
 
@@ -420,7 +425,7 @@ def main():
                           cfl_num=0.35, nu=1e-4, kappa=1e-4, sigma=1.0, g=1.0, rho0=1.0, rho0_prime=1.0,
                           rho_eps1=1e-2, noise=0, linear=0)
             t = g.autotune(reps=20)
-            names = ["p1_row", "p1_tile", "p1_pair", "p2_three", "p2_four", "p3_pair", "p3_tile"]
+            names = ["p1_row", "p1_tile", "p1_pair", "p2_three", "p2_four", "p3_pair", "p3_tile", "p3_split"]
             print(json.dumps({"n": n, "precision": args.precision, "choice": [t["p1"], t["p2"], t["p3"]],
                               "us_per_launch": {k: round(v * 1e3, 2) for k, v in zip(names, t["ms"])},
                               "fuse": t["fuse"], "us_per_step": {"unfused": round(t["step_ms"][0] * 1e3, 2),
@@ -464,7 +469,7 @@ def main():
     dom = int(np.argmax(kt))
     dname = names[dom]
     achieved = kb[dname] / (kt[dom] * 1e-3) / 1e9
-    traffic = read_traffic(f"{dname}_{n}_{args.precision}")
+    traffic, traffic_src = read_traffic(f"{dname}_{n}_{args.precision}")
     ms_step = ms / args.steps
     step_bytes = sum(kb.values())
     baseline_formula = c * (14 * s["M"] + 10 * s["I"])
@@ -488,6 +493,7 @@ def main():
                    "variants": f"{used['p1']},{used['p2']},{used['p3']}" + ("" if args.variants else " (default)")},
         "roofline": {"bound": "hbm", "kernel": dname, "achieved": achieved, "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "bytes_per_launch": kb[dname], "ms_per_launch": float(kt[dom])},
         "kernels_ms": {k: float(v) for k, v in zip(names, kt)},
         "stream_copy_GBs": copy_bw,

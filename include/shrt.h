@@ -234,12 +234,13 @@ shrt_status shrt_stream_copy_bench(long long bytes, int reps, double* ms);
  * kernel variants: P1 one (row pair, output) per workgroup (0), a tile of four
  * row pairs (1) or one (row pair, field group) with its two outputs (2); P2
  * three x-inverse transforms per column (0) or four (1); P3 one row pair per
- * workgroup (0) or four (1).  shrt_autotune times the variants of each pass on
+ * workgroup (0), four (1) or one row of a pair (2: twice the workgroups, each
+ * transforming the pair's H row and updating its own row).  shrt_autotune times the variants of each pass on
  * a scratch instance of the context's shape and parameters (reps launches
  * each) and keeps the fastest.  The variants agree to rounding (the P2
  * variants pair the fields differently in the complex transforms; all are
  * compiled separately).  ms[] = average ms per launch of p1 row, p1 tile,
- * p1 pair, p2 three, p2 four, p3 pair, p3 tile.  It then times
+ * p1 pair, p2 three, p2 four, p3 pair, p3 tile, p3 split.  It then times
  * captured batches of graph_steps steps (20 if the context has none) with and
  * without the P3+P1 fusion (shrt_set_fusion) and keeps the faster.  Without
  * autotuning a context starts with P1 variant 2 for ny >= 2048, 1 for ny = 512
@@ -247,7 +248,7 @@ shrt_status shrt_stream_copy_bench(long long bytes, int reps, double* ms);
  * shrt_autotune on MI355X.  shrt_get_variants reports the current choice. */
 typedef struct {
     int p1, p2, p3;
-    double ms[7];
+    double ms[8];
     int fuse;            /* 1: captured batches fuse P3 with the next P1 (shrt_set_fusion) */
     double step_ms[2];   /* ms per step of captured batches, unfused and fused */
 } shrt_tuning;

@@ -757,7 +757,7 @@ shrt_status shrt_autotune(shrt_ctx* c, int reps, shrt_tuning* out) {
     shrt_seeded_ic ic{12345u, 2.0, std::max(2.0, c->g.ny / 16.0), 1e-4, -1.5};
     st = shrt_initialize_seeded(t, &ic);
     if (st == SHRT_OK) st = shrt_step(t, 4);
-    static const int nvar[3] = {3, 2, 2};
+    static const int nvar[3] = {3, 2, 3};
     double ms[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     const bool tiles = c->prec == 4 ? Launch<float>::tiles_supported(c->g.ny) : Launch<double>::tiles_supported(c->g.ny);
     for (int pass = 0; pass < 3 && st == SHRT_OK; ++pass)
@@ -774,7 +774,9 @@ shrt_status shrt_autotune(shrt_ctx* c, int reps, shrt_tuning* out) {
     for (int v = 1; v < nvar[0]; ++v)
         if (ms[0][v] < ms[0][chosen.p1]) chosen.p1 = v;
     chosen.p2 = ms[1][1] < ms[1][0] ? kP2Four : kP2Three;
-    chosen.p3 = ms[2][1] < ms[2][0] ? kP3Tile : kP3Pair;
+    chosen.p3 = kP3Pair;
+    for (int v = 1; v < nvar[2]; ++v)
+        if (ms[2][v] < ms[2][chosen.p3]) chosen.p3 = v;
     // captured batches with and without the P3+P1 fusion, alternated, best of two each
     double step_ms[2] = {1e300, 1e300};
     const int nb = std::max(2, reps / 4);
@@ -811,7 +813,7 @@ shrt_status shrt_autotune(shrt_ctx* c, int reps, shrt_tuning* out) {
 }
 
 shrt_status shrt_set_variants(shrt_ctx* c, int p1, int p2, int p3) {
-    if (!c || p1 < 0 || p1 > 2 || p2 < 0 || p2 > 1 || p3 < 0 || p3 > 1) return fail(SHRT_ERR_INVALID, "bad variant");
+    if (!c || p1 < 0 || p1 > 2 || p2 < 0 || p2 > 1 || p3 < 0 || p3 > 2) return fail(SHRT_ERR_INVALID, "bad variant");
     const bool tiles = c->prec == 4 ? Launch<float>::tiles_supported(c->g.ny) : Launch<double>::tiles_supported(c->g.ny);
     if (!tiles && (p1 == kP1Tile || p3 == kP3Tile))
         return fail(SHRT_ERR_UNSUPPORTED, "the four-team tile variants do not fit this grid's LDS (FP64 4096)");

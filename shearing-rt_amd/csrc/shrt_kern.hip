@@ -152,6 +152,15 @@ hipError_t Launch<R>::p3(const StepArgs<R>& a, const StepArgs<R>* dargs, int cou
         });
         return ok ? hipGetLastError() : unsupported();
     }
+    if (variant == kP3Split) {   // one row of a row pair per workgroup
+        const bool ok = dispatch(a.g.ny, [&](auto nc) {
+            constexpr int N = decltype(nc)::value;
+            const dim3 grid(4 * a.g.nkxh, dargs ? count : 1), block(TeamShape<N>::TT);
+            if (dargs) step_launch((k_p3<R, N, true, 1, false, false, true>), grid, block, 0, s, a, dargs);
+            else step_launch((k_p3<R, N, false, 1, false, false, true>), grid, block, 0, s, a, dargs);
+        });
+        return ok ? hipGetLastError() : unsupported();
+    }
     const bool ok = variant == kP3Pair ? go(std::integral_constant<int, 1>{}) : go(std::integral_constant<int, kTile>{});
     return ok ? hipGetLastError() : unsupported();
 }

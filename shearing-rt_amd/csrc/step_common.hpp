@@ -46,7 +46,8 @@ template <typename T, int N> constexpr int p3_points() { return TeamShape<N>::P;
 enum : int { kP1Row = 0, kP1Tile = 1, kP1Pair = 2 };   // per workgroup: one (row pair, output) / a tile of kTile
                                                        // row pairs / one (row pair, field group): two outputs
 enum : int { kP2Three = 0, kP2Four = 1 };      // x-inverse transforms per column
-enum : int { kP3Pair = 0, kP3Tile = 1 };       // one row pair / kTile row pairs per workgroup
+enum : int { kP3Pair = 0, kP3Tile = 1, kP3Split = 2 };   // one row pair / kTile row pairs / one row of a pair
+                                                         // per workgroup
 struct Variants {
     int p1 = kP1Row, p2 = kP2Three, p3 = kP3Pair;
     int fuse = 0;   // captured batches run P3 of step n and P1 of step n+1 as one kernel

@@ -814,7 +814,11 @@ __device__ void p3_bookkeeping(const StepArgs<T>& a, double dt, int h, int which
 // the same FP64 operations as the bookkeeping, and the outputs of this field
 // (omega: vx, vy, d/dx, d/dy; rho: d/dx, d/dy) are transformed and written to
 // G through the remap view, as k_p1_row would after reading the rows back.
-template <typename T, int NY, bool BATCH, int KT, bool FUSE, bool NARROW = false>
+// SPLIT: two workgroups per (row pair, field), one per row of the pair (side
+// 0: kx, side 1: -kx); each transforms the H row (the transform gives both
+// rows) and updates its own row only.  Twice the workgroups for grids whose
+// row pairs alone do not fill the chip; the same arithmetic per mode.
+template <typename T, int NY, bool BATCH, int KT, bool FUSE, bool NARROW = false, bool SPLIT = false>
 __global__ void __launch_bounds__((KT * TeamShape<NY>::TT), (waves_per_eu<T, NY, TeamShape<NY>::TT, KT>()))
 k_p3(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
     const StepArgs<T>& a = step_args_of<BATCH>(a1, args);
@@ -830,6 +834,7 @@ k_p3(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
     constexpr int JB = (sizeof(T) == 4 && !BATCH && KT == 1 && !FUSE) ? 3 : 2;
     static_assert(KT == 1 || KT == kTile, "P3 workgroups hold one row pair or one H tile");
     static_assert(!FUSE || KT == 1, "the fused next-step P1 works on one row pair");
+    static_assert(!SPLIT || (KT == 1 && !FUSE), "split workgroups hold one row of one pair");
     __shared__ cplx<T> sm[KT * L];
     __shared__ double red[32];
     __shared__ cplx<T> slot[2];   // FUSE: the new ky = 0 values of the row and of its mirror
@@ -837,7 +842,9 @@ k_p3(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
     const Phys& ph = a.ph;
     const int team = threadIdx.x / TT, tid = threadIdx.x - team * TT;
     // the omega and rho workgroups of a row group read each other's rows: keep them on one XCD
-    const int q = xcd_item(blockIdx.x, gridDim.x);
+    const int qq = xcd_item(blockIdx.x, gridDim.x);
+    const int myside = SPLIT ? (qq & 1) : 0;   // SPLIT: the row of the pair this workgroup updates
+    const int q = SPLIT ? (qq >> 1) : qq;
     const int tile = q >> 1;
     const int f = q & 1;   // 0: omega, 1: rho
     const int row = tile * KT + team;
@@ -892,7 +899,8 @@ k_p3(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
         else return buf[(long long)r * g.nky + ik];
     };
     auto load_batch = [&](auto view, auto bc, Batch& B) {
-        constexpr int b = decltype(bc)::value, side = b / NBS, j0 = (b % NBS) * JB;
+        constexpr int b = decltype(bc)::value, j0 = (b % NBS) * JB;
+        const int side = SPLIT ? myside : b / NBS;
         if (!act || (side == 1 && row == 0)) return;
         const int r = side ? rn : row;
 #pragma unroll
@@ -909,11 +917,12 @@ k_p3(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
         }
     };
     auto update_batch = [&](auto bc, const Batch& B) {
-        constexpr int b = decltype(bc)::value, side = b / NBS, j0 = (b % NBS) * JB;
+        constexpr int b = decltype(bc)::value, j0 = (b % NBS) * JB;
+        const int side = SPLIT ? myside : b / NBS;
         if (!act || (side == 1 && row == 0)) return;
         const int r = side ? rn : row;
-        const double kxv = kx_r[side];
-        const double sr = s_r[side];
+        const double kxv = side ? kx_r[1] : kx_r[0];   // no dynamic index into the two-element arrays
+        const double sr = side ? s_r[1] : s_r[0];
         // add_ddxrho: g/rho0 * kx;  add_ddxphi: rho0' * kx
         const T cc = (T)mul_rn(f == 0 ? ph.g / ph.rho0 : ph.rho0_prime, kxv);
 #pragma unroll
@@ -1002,9 +1011,10 @@ k_p3(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
             load_batch(view, std::integral_constant<int, 0>{}, bt[0]);
         }
 
-        static_for<0, NB>([&](auto bc) {
+        constexpr int NBT = SPLIT ? NBS : NB;   // the batches of this workgroup's row(s)
+        static_for<0, NBT>([&](auto bc) {
             constexpr int b = decltype(bc)::value;
-            if constexpr (b + 1 < NB) load_batch(view, std::integral_constant<int, b + 1>{}, bt[(b + 1) & 1]);
+            if constexpr (b + 1 < NBT) load_batch(view, std::integral_constant<int, b + 1>{}, bt[(b + 1) & 1]);
             asm volatile("" ::: "memory");   // keep those loads ahead of this batch's stores
             update_batch(bc, bt[b & 1]);
         });

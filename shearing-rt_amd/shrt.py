@@ -60,7 +60,7 @@ class Diag(ctypes.Structure):
 
 
 class Tuning(ctypes.Structure):
-    _fields_ = [("p1", ctypes.c_int), ("p2", ctypes.c_int), ("p3", ctypes.c_int), ("ms", ctypes.c_double * 7),
+    _fields_ = [("p1", ctypes.c_int), ("p2", ctypes.c_int), ("p3", ctypes.c_int), ("ms", ctypes.c_double * 8),
                 ("fuse", ctypes.c_int), ("step_ms", ctypes.c_double * 2)]
 
 

@@ -734,7 +734,7 @@ def test_kernel_variants(gpu, prec):
     o.step(steps)
     ref = o.buffer("arho")
     outs = {}
-    for v in [(a, b, c) for a in (0, 1, 2) for b in (0, 1) for c in (0, 1)]:
+    for v in [(a, b, c) for a in (0, 1, 2) for b in (0, 1) for c in (0, 1, 2)]:
         g = shrt.Shrt(nx=n, ny=n, precision=prec, graph_steps=4, **dict(BASE, sigma=1.0))
         g.set_variants(*v)
         g.initialize_seeded(**ic)
@@ -744,9 +744,13 @@ def test_kernel_variants(gpu, prec):
         g.close()
     for v, r in outs.items():
         assert rel_l2(r, outs[(0, 0, 0)]) < TOLN[prec], v
+    # one row of a pair per P3 workgroup: the same arithmetic per mode as the pair form
+    for a in (0, 1, 2):
+        for b in (0, 1):
+            assert np.array_equal(outs[(a, b, 2)].view(np.uint8), outs[(a, b, 0)].view(np.uint8)), (a, b)
     g = shrt.Shrt(nx=n, ny=n, precision=prec, graph_steps=4, **dict(BASE, sigma=1.0))
     t = g.autotune(reps=5)
-    assert t["p1"] in (0, 1, 2) and t["p2"] in (0, 1) and t["p3"] in (0, 1) and t["fuse"] in (0, 1)
+    assert t["p1"] in (0, 1, 2) and t["p2"] in (0, 1) and t["p3"] in (0, 1, 2) and t["fuse"] in (0, 1)
     assert all(m > 0 for m in t["ms"]) and all(m > 0 for m in t["step_ms"])
     assert t["fuse"] == (t["step_ms"][1] < t["step_ms"][0])
     g.initialize_seeded(**ic)     # the tuned context steps (fused or not) and matches the oracle
