@@ -27,9 +27,9 @@ struct AuxArgs {
 
 // x-forward of one real field f[ix*ny + iy] for a tile of 4 y columns -> H tile.
 template <typename T, int NX>
-__global__ void __launch_bounds__(kTile * TeamShape<NX>::TT)
+__global__ void __launch_bounds__((kTile * TeamShape<T, NX>::TT))
 k_aux_xfwd(AuxArgs<T> a) {
-    using S = TeamShape<NX>;
+    using S = TeamShape<T, NX>;
     constexpr int P = S::P, TT = S::TT, L = lds_len<NX>();
     constexpr int NQ = (NX / 3 + 1 + TT - 1) / TT;
     __shared__ cplx<T> sm[kTile * L];
@@ -66,9 +66,9 @@ k_aux_xfwd(AuxArgs<T> a) {
 // y-forward of an H field for a tile of kx>=0 rows -> F (reference layout),
 // truncated and scaled (scale_dealias).
 template <typename T, int NY>
-__global__ void __launch_bounds__(kTile * TeamShape<NY>::TT)
+__global__ void __launch_bounds__((kTile * TeamShape<T, NY>::TT))
 k_aux_yfwd(AuxArgs<T> a) {
-    using S = TeamShape<NY>;
+    using S = TeamShape<T, NY>;
     constexpr int P = S::P, TT = S::TT, L = lds_len<NY>();
     __shared__ cplx<T> sm[kTile * L];
     const Geo& g = a.g;
@@ -111,9 +111,9 @@ k_aux_yfwd(AuxArgs<T> a) {
 // y-inverse of a spectral field (reference layout, through the remap view),
 // times the multiplier of `mode`, for a tile of kx>=0 rows -> G tile.
 template <typename T, int NY>
-__global__ void __launch_bounds__(kTile * TeamShape<NY>::TT)
+__global__ void __launch_bounds__((kTile * TeamShape<T, NY>::TT))
 k_aux_yinv(AuxArgs<T> a) {
-    using S = TeamShape<NY>;
+    using S = TeamShape<T, NY>;
     constexpr int P = S::P, TT = S::TT, L = lds_len<NY>();
     __shared__ cplx<T> sm[kTile * L];
     const Geo& g = a.g;
@@ -171,9 +171,9 @@ k_aux_yinv(AuxArgs<T> a) {
 
 // x-inverse (C2R) of a G field for a tile of 4 y columns -> real f[ix*ny + iy].
 template <typename T, int NX>
-__global__ void __launch_bounds__(kTile * TeamShape<NX>::TT)
+__global__ void __launch_bounds__((kTile * TeamShape<T, NX>::TT))
 k_aux_xinv(AuxArgs<T> a) {
-    using S = TeamShape<NX>;
+    using S = TeamShape<T, NX>;
     constexpr int P = S::P, TT = S::TT, L = lds_len<NX>();
     __shared__ cplx<T> sm[kTile * L];
     const Geo& g = a.g;
@@ -212,9 +212,9 @@ k_aux_xinv(AuxArgs<T> a) {
 // yy[iy] = iy*dy, formed in FP64.  Output tiles [t][row][c] (y = 4t + c), row
 // stride nkx.
 template <typename T, int NY>
-__global__ void __launch_bounds__(kTile * TeamShape<NY>::TT)
+__global__ void __launch_bounds__((kTile * TeamShape<T, NY>::TT))
 k_aux_yinv_lab(AuxArgs<T> a) {
-    using S = TeamShape<NY>;
+    using S = TeamShape<T, NY>;
     constexpr int P = S::P, TT = S::TT, L = lds_len<NY>();
     constexpr int LP = stage_stride<NY>();
     __shared__ cplx<T> sm[kTile * L];
@@ -260,9 +260,9 @@ k_aux_yinv_lab(AuxArgs<T> a) {
 // fft.cu:208-250): per y column, the stored kx rows padded to NX, a complex
 // inverse FFT, and the real part -> f[ix*ny + iy].
 template <typename T, int NX>
-__global__ void __launch_bounds__(kTile * TeamShape<NX>::TT)
+__global__ void __launch_bounds__((kTile * TeamShape<T, NX>::TT))
 k_aux_xinv_lab(AuxArgs<T> a) {
-    using S = TeamShape<NX>;
+    using S = TeamShape<T, NX>;
     constexpr int P = S::P, TT = S::TT, L = lds_len<NX>();
     __shared__ cplx<T> sm[kTile * L];
     const Geo& g = a.g;

@@ -565,10 +565,11 @@ shrt_status shrt_create(const shrt_params* p, shrt_ctx** out) {
     ph.rho0_prime = p->rho0_prime; ph.cfl_num = p->cfl_num; ph.eps = 1e-10; ph.linear = p->linear ? 1 : 0;
     ph.narrow = 0;
     // Default P1 variant by the y length, as shrt_autotune picks it on MI355X:
-    // one (row pair, field group) per workgroup from 2048 up, the four-row tile
-    // at 512 and at 1024 in FP32, one (row pair, output) otherwise.
-    if (g.ny >= 2048) c->var.p1 = kP1Pair;
-    else if (g.ny == 512 || (g.ny == 1024 && c->prec == 4)) c->var.p1 = kP1Tile;
+    // one (row pair, field group) per workgroup at 2048 in FP32 and at 4096, the
+    // four-row tile at 512, one (row pair, output) otherwise (the 8-point teams
+    // of 1024 and of FP64 2048 fill the chip with one output per workgroup).
+    if (g.ny >= 4096 || (g.ny == 2048 && c->prec == 4)) c->var.p1 = kP1Pair;
+    else if (g.ny == 512) c->var.p1 = kP1Tile;
 
     auto bail = [&](shrt_status st) {
         shrt_destroy(c);

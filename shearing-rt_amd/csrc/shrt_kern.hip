@@ -83,7 +83,7 @@ int Launch<R>::p2_workgroups(int nx, int ny) {
 template <>
 int Launch<R>::team_threads(int n) {
     int tt = 0;
-    dispatch(n, [&](auto nc) { tt = TeamShape<decltype(nc)::value>::TT; });
+    dispatch(n, [&](auto nc) { tt = TeamShape<R, decltype(nc)::value>::TT; });
     return tt;
 }
 
@@ -93,7 +93,7 @@ hipError_t Launch<R>::p1(const StepArgs<R>& a, const StepArgs<R>* dargs, int cou
         const bool ok = dispatch<kTile>(a.g.ny, [&](auto nc) {
             constexpr int N = decltype(nc)::value;
             const P1Grid pg(a.g.nkxh, a.ph.linear);
-            const dim3 grid(pg.grid, dargs ? count : 1), block(kTile * TeamShape<N>::TT);
+            const dim3 grid(pg.grid, dargs ? count : 1), block(kTile * TeamShape<R, N>::TT);
             if (dargs) step_launch((k_p1<R, N, true>), grid, block, 0, s, a, dargs);
             else step_launch((k_p1<R, N, false>), grid, block, 0, s, a, dargs);
         });
@@ -103,12 +103,12 @@ hipError_t Launch<R>::p1(const StepArgs<R>& a, const StepArgs<R>* dargs, int cou
         constexpr int N = decltype(nc)::value;
         if (variant == kP1Pair) {
             const dim3 grid(a.g.nkxh * (a.ph.linear ? 1 : 3), dargs ? count : 1);
-            if (dargs) step_launch((k_p1_pair<R, N, true>), grid, dim3(TeamShape<N>::TT), 0, s, a, dargs);
-            else step_launch((k_p1_pair<R, N, false>), grid, dim3(TeamShape<N>::TT), 0, s, a, dargs);
+            if (dargs) step_launch((k_p1_pair<R, N, true>), grid, dim3(TeamShape<R, N>::TT), 0, s, a, dargs);
+            else step_launch((k_p1_pair<R, N, false>), grid, dim3(TeamShape<R, N>::TT), 0, s, a, dargs);
         } else if (variant == kP1Row) {
             const dim3 grid((a.g.nkxh - 1) * (a.ph.linear ? 2 : 6) + (a.ph.linear ? 1 : 3), dargs ? count : 1);
-            if (dargs) step_launch((k_p1_row<R, N, true>), grid, dim3(TeamShape<N>::TT), 0, s, a, dargs);
-            else step_launch((k_p1_row<R, N, false>), grid, dim3(TeamShape<N>::TT), 0, s, a, dargs);
+            if (dargs) step_launch((k_p1_row<R, N, true>), grid, dim3(TeamShape<R, N>::TT), 0, s, a, dargs);
+            else step_launch((k_p1_row<R, N, false>), grid, dim3(TeamShape<R, N>::TT), 0, s, a, dargs);
         }
     });
     return ok ? hipGetLastError() : unsupported();
@@ -138,7 +138,7 @@ hipError_t Launch<R>::p3(const StepArgs<R>& a, const StepArgs<R>* dargs, int cou
         constexpr int KT = decltype(ktc)::value;
         return dispatch<KT>(a.g.ny, [&](auto nc) {
             constexpr int N = decltype(nc)::value;
-            const dim3 grid(2 * ((a.g.nkxh + KT - 1) / KT), dargs ? count : 1), block(KT * TeamShape<N>::TT);
+            const dim3 grid(2 * ((a.g.nkxh + KT - 1) / KT), dargs ? count : 1), block(KT * TeamShape<R, N>::TT);
             if (dargs) step_launch((k_p3<R, N, true, KT, false>), grid, block, 0, s, a, dargs);
             else step_launch((k_p3<R, N, false, KT, false>), grid, block, 0, s, a, dargs);
         });
@@ -146,7 +146,7 @@ hipError_t Launch<R>::p3(const StepArgs<R>& a, const StepArgs<R>* dargs, int cou
     if (sizeof(R) == 4 && a.ph.narrow) {   // the FP32-arithmetic update: one row pair per workgroup
         const bool ok = dispatch(a.g.ny, [&](auto nc) {
             constexpr int N = decltype(nc)::value;
-            const dim3 grid(2 * a.g.nkxh, dargs ? count : 1), block(TeamShape<N>::TT);
+            const dim3 grid(2 * a.g.nkxh, dargs ? count : 1), block(TeamShape<R, N>::TT);
             if (dargs) step_launch((k_p3<R, N, true, 1, false, true>), grid, block, 0, s, a, dargs);
             else step_launch((k_p3<R, N, false, 1, false, true>), grid, block, 0, s, a, dargs);
         });
@@ -155,7 +155,7 @@ hipError_t Launch<R>::p3(const StepArgs<R>& a, const StepArgs<R>* dargs, int cou
     if (variant == kP3Split) {   // one row of a row pair per workgroup
         const bool ok = dispatch(a.g.ny, [&](auto nc) {
             constexpr int N = decltype(nc)::value;
-            const dim3 grid(4 * a.g.nkxh, dargs ? count : 1), block(TeamShape<N>::TT);
+            const dim3 grid(4 * a.g.nkxh, dargs ? count : 1), block(TeamShape<R, N>::TT);
             if (dargs) step_launch((k_p3<R, N, true, 1, false, false, true>), grid, block, 0, s, a, dargs);
             else step_launch((k_p3<R, N, false, 1, false, false, true>), grid, block, 0, s, a, dargs);
         });
@@ -170,7 +170,7 @@ hipError_t Launch<R>::p31(const StepArgs<R>& a, const StepArgs<R>* dargs, int co
     if (a.ph.narrow) return unsupported();   // the FP32-arithmetic update runs unfused
     bool ok = dispatch(a.g.ny, [&](auto nc) {
         constexpr int N = decltype(nc)::value;
-        const dim3 grid(2 * a.g.nkxh, dargs ? count : 1), block(TeamShape<N>::TT);
+        const dim3 grid(2 * a.g.nkxh, dargs ? count : 1), block(TeamShape<R, N>::TT);
         if (dargs) step_launch((k_p3<R, N, true, 1, true>), grid, block, 0, s, a, dargs);
         else step_launch((k_p3<R, N, false, 1, true>), grid, block, 0, s, a, dargs);
     });
@@ -181,7 +181,7 @@ template <>
 hipError_t Launch<R>::xfwd(const AuxArgs<R>& a, hipStream_t s) {
     bool ok = dispatch<kTile>(a.g.nx, [&](auto nc) {
         constexpr int N = decltype(nc)::value;
-        hipLaunchKernelGGL((k_aux_xfwd<R, N>), dim3(a.g.ntile_y), dim3(kTile * TeamShape<N>::TT), 0, s, a);
+        hipLaunchKernelGGL((k_aux_xfwd<R, N>), dim3(a.g.ntile_y), dim3(kTile * TeamShape<R, N>::TT), 0, s, a);
     });
     return ok ? hipGetLastError() : unsupported();
 }
@@ -190,7 +190,7 @@ template <>
 hipError_t Launch<R>::yfwd(const AuxArgs<R>& a, hipStream_t s) {
     bool ok = dispatch<kTile>(a.g.ny, [&](auto nc) {
         constexpr int N = decltype(nc)::value;
-        hipLaunchKernelGGL((k_aux_yfwd<R, N>), dim3(a.g.ntile_x), dim3(kTile * TeamShape<N>::TT), 0, s, a);
+        hipLaunchKernelGGL((k_aux_yfwd<R, N>), dim3(a.g.ntile_x), dim3(kTile * TeamShape<R, N>::TT), 0, s, a);
     });
     return ok ? hipGetLastError() : unsupported();
 }
@@ -199,7 +199,7 @@ template <>
 hipError_t Launch<R>::yinv(const AuxArgs<R>& a, hipStream_t s) {
     bool ok = dispatch<kTile>(a.g.ny, [&](auto nc) {
         constexpr int N = decltype(nc)::value;
-        hipLaunchKernelGGL((k_aux_yinv<R, N>), dim3(a.g.ntile_x), dim3(kTile * TeamShape<N>::TT), 0, s, a);
+        hipLaunchKernelGGL((k_aux_yinv<R, N>), dim3(a.g.ntile_x), dim3(kTile * TeamShape<R, N>::TT), 0, s, a);
     });
     return ok ? hipGetLastError() : unsupported();
 }
@@ -208,7 +208,7 @@ template <>
 hipError_t Launch<R>::xinv(const AuxArgs<R>& a, hipStream_t s) {
     bool ok = dispatch<kTile>(a.g.nx, [&](auto nc) {
         constexpr int N = decltype(nc)::value;
-        hipLaunchKernelGGL((k_aux_xinv<R, N>), dim3(a.g.ntile_y), dim3(kTile * TeamShape<N>::TT), 0, s, a);
+        hipLaunchKernelGGL((k_aux_xinv<R, N>), dim3(a.g.ntile_y), dim3(kTile * TeamShape<R, N>::TT), 0, s, a);
     });
     return ok ? hipGetLastError() : unsupported();
 }
@@ -218,7 +218,7 @@ hipError_t Launch<R>::yinv_lab(const AuxArgs<R>& a, hipStream_t s) {
     bool ok = dispatch<kTile>(a.g.ny, [&](auto nc) {
         constexpr int N = decltype(nc)::value;
         hipLaunchKernelGGL((k_aux_yinv_lab<R, N>), dim3((a.g.nkx + kTile - 1) / kTile),
-                           dim3(kTile * TeamShape<N>::TT), 0, s, a);
+                           dim3(kTile * TeamShape<R, N>::TT), 0, s, a);
     });
     return ok ? hipGetLastError() : unsupported();
 }
@@ -227,7 +227,7 @@ template <>
 hipError_t Launch<R>::xinv_lab(const AuxArgs<R>& a, hipStream_t s) {
     bool ok = dispatch<kTile>(a.g.nx, [&](auto nc) {
         constexpr int N = decltype(nc)::value;
-        hipLaunchKernelGGL((k_aux_xinv_lab<R, N>), dim3(a.g.ntile_y), dim3(kTile * TeamShape<N>::TT), 0, s, a);
+        hipLaunchKernelGGL((k_aux_xinv_lab<R, N>), dim3(a.g.ntile_y), dim3(kTile * TeamShape<R, N>::TT), 0, s, a);
     });
     return ok ? hipGetLastError() : unsupported();
 }

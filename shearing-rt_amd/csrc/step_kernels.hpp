@@ -107,7 +107,7 @@ __host__ __device__ inline int p1_row0_output(int j) { return j == 0 ? 0 : (j ==
 template <int OI, typename T, int NY>
 __device__ __forceinline__ void p1_spectrum(const StepArgs<T>& a, int row, bool act, int tid, cplx<T>* v) {
     constexpr int GRP = OI / 2, O = OI % 2;
-    using S = TeamShape<NY>;
+    using S = TeamShape<T, NY>;
     constexpr int P = S::P, TT = S::TT;
     const Geo& g = a.g;
     const int rowc = act ? row : 0;
@@ -143,7 +143,7 @@ __device__ __forceinline__ void p1_spectrum(const StepArgs<T>& a, int row, bool 
 // pair (row, -row), by one team.
 template <typename T, int NY>
 __device__ __forceinline__ void p1_store_phi(const StepArgs<T>& a, int row, int tid) {
-    constexpr int TT = TeamShape<NY>::TT;
+    constexpr int TT = TeamShape<T, NY>::TT;
     const Geo& g = a.g;
     const StepCtl* c = a.ctl_in;
     const int s1 = slot_a1(c->istep);
@@ -169,10 +169,10 @@ __device__ __forceinline__ void p1_store_phi(const StepArgs<T>& a, int row, int 
 // written by the team directly; the first of them also writes the vanishing
 // outputs of row 0.
 template <typename T, int NY, bool BATCH>
-__global__ void __launch_bounds__(kTile * TeamShape<NY>::TT, (waves_per_eu<T, NY>()))
+__global__ void __launch_bounds__((kTile * TeamShape<T, NY>::TT), (waves_per_eu<T, NY>()))
 k_p1(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
     const StepArgs<T>& a = step_args_of<BATCH>(a1, args);
-    using S = TeamShape<NY>;
+    using S = TeamShape<T, NY>;
     constexpr int P = S::P, TT = S::TT, L = lds_len<NY>();
     constexpr int LP = stage_stride<NY>();
     __shared__ cplx<T> sm[kTile * L];
@@ -251,10 +251,10 @@ k_p1(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
 // t; the outputs of neighbouring rows are dispatched together to one XCD, whose
 // L2 merges the runs into whole lines.
 template <typename T, int NY, bool BATCH>
-__global__ void __launch_bounds__(TeamShape<NY>::TT, (waves_per_eu<T, NY, TeamShape<NY>::TT, 1>()))
+__global__ void __launch_bounds__((TeamShape<T, NY>::TT), (waves_per_eu<T, NY, TeamShape<T, NY>::TT, 1>()))
 k_p1_row(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
     const StepArgs<T>& a = step_args_of<BATCH>(a1, args);
-    using S = TeamShape<NY>;
+    using S = TeamShape<T, NY>;
     constexpr int P = S::P, TT = S::TT, L = lds_len<NY>();
     __shared__ cplx<T> sm[L];
     const Geo& g = a.g;
@@ -305,7 +305,7 @@ template <typename T, int NY>
 __device__ __forceinline__ void p1_outputs(const StepArgs<T>& a, int row, const cplx<T>* base, cplx<T> base_n0,
                                            const double* sn, const double* kxr, int o_begin, int o_end,
                                            int store_phi, cplx<T>* lds, int tid) {
-    using S = TeamShape<NY>;
+    using S = TeamShape<T, NY>;
     constexpr int P = S::P, TT = S::TT;
     constexpr int NKY = NY / 3 + 1;
     const Geo& g = a.g;
@@ -373,10 +373,10 @@ __device__ __forceinline__ void p1_outputs(const StepArgs<T>& a, int row, const 
 // through the remap view and both outputs of the group are transformed from
 // it; 3 * nkxh workgroups (nkxh in the linear mode, group 0 only).
 template <typename T, int NY, bool BATCH>
-__global__ void __launch_bounds__(TeamShape<NY>::TT, (waves_per_eu<T, NY, TeamShape<NY>::TT, 1>()))
+__global__ void __launch_bounds__((TeamShape<T, NY>::TT), (waves_per_eu<T, NY, TeamShape<T, NY>::TT, 1>()))
 k_p1_pair(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
     const StepArgs<T>& a = step_args_of<BATCH>(a1, args);
-    using S = TeamShape<NY>;
+    using S = TeamShape<T, NY>;
     constexpr int P = S::P, TT = S::TT, L = lds_len<NY>();
     constexpr int NKY = NY / 3 + 1;
     __shared__ cplx<T> sm[L];
@@ -819,10 +819,10 @@ __device__ void p3_bookkeeping(const StepArgs<T>& a, double dt, int h, int which
 // rows) and updates its own row only.  Twice the workgroups for grids whose
 // row pairs alone do not fill the chip; the same arithmetic per mode.
 template <typename T, int NY, bool BATCH, int KT, bool FUSE, bool NARROW = false, bool SPLIT = false>
-__global__ void __launch_bounds__((KT * TeamShape<NY>::TT), (waves_per_eu<T, NY, TeamShape<NY>::TT, KT>()))
+__global__ void __launch_bounds__((KT * TeamShape<T, NY>::TT), (waves_per_eu<T, NY, TeamShape<T, NY>::TT, KT>()))
 k_p3(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
     const StepArgs<T>& a = step_args_of<BATCH>(a1, args);
-    using S = TeamShape<NY>;
+    using S = TeamShape<T, NY>;
     constexpr int P = S::P, TT = S::TT, L = lds_len<NY>();
     constexpr int NT = KT * TT;
     constexpr int LP = stage_stride<NY>();
