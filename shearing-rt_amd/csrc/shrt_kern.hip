@@ -102,7 +102,7 @@ hipError_t Launch<R>::p1(const StepArgs<R>& a, const StepArgs<R>* dargs, int cou
     bool ok = dispatch(a.g.ny, [&](auto nc) {
         constexpr int N = decltype(nc)::value;
         if (variant == kP1Pair) {
-            const dim3 grid(a.g.nkxh * (a.ph.linear ? 1 : 3), dargs ? count : 1);
+            const dim3 grid(a.ph.linear ? a.g.nkxh : 3 * (a.g.nkxh - 1) + 2, dargs ? count : 1);
             if (dargs) step_launch((k_p1_pair<R, N, true>), grid, dim3(TeamShape<R, N>::TT), 0, s, a, dargs);
             else step_launch((k_p1_pair<R, N, false>), grid, dim3(TeamShape<R, N>::TT), 0, s, a, dargs);
         } else if (variant == kP1Row) {
@@ -119,7 +119,6 @@ hipError_t Launch<R>::p2(const StepArgs<R>& a, const StepArgs<R>* dargs, int cou
     bool ok = dispatch(a.g.nx, [&](auto nc) {
         constexpr int N = decltype(nc)::value;
         constexpr int KT = p2_teams<R, N>();
-        static_assert(KT == 1, "the P2 length dispatch assumes one team per workgroup");
         const dim3 grid(a.g.ny / KT, dargs ? count : 1), block(KT * Team<N, p2_points<R, N>()>::TT);
         if (variant == kP2Three) {
             if (dargs) step_launch((k_p2<R, N, true, true>), grid, block, 0, s, a, dargs);
@@ -170,7 +169,7 @@ hipError_t Launch<R>::p31(const StepArgs<R>& a, const StepArgs<R>* dargs, int co
     if (a.ph.narrow) return unsupported();   // the FP32-arithmetic update runs unfused
     bool ok = dispatch(a.g.ny, [&](auto nc) {
         constexpr int N = decltype(nc)::value;
-        const dim3 grid(2 * a.g.nkxh, dargs ? count : 1), block(TeamShape<R, N>::TT);
+        const dim3 grid(3 * (a.g.nkxh - 1) + 2, dargs ? count : 1), block(TeamShape<R, N>::TT);
         if (dargs) step_launch((k_p3<R, N, true, 1, true>), grid, block, 0, s, a, dargs);
         else step_launch((k_p3<R, N, false, 1, true>), grid, block, 0, s, a, dargs);
     });

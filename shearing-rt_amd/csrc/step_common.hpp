@@ -36,8 +36,12 @@ constexpr int waves_per_eu() {
 // (FP64 2048^2: 77.5 us per launch against 87.2 us with the 8 points of its
 // P1/P3 teams), the P1/P3 team shape below.
 template <typename T, int N> constexpr int p2_points() { return N >= 2048 ? 16 : TeamShape<T, N>::P; }
-// Columns (teams) per P2 workgroup: 1, 2 or kTile.
-template <typename T, int N> constexpr int p2_teams() { return 1; }
+// Columns (teams) per P2 workgroup: 1, 2 or kTile.  Two at FP64 2048 (P2
+// 57.9 us per launch against 66.5 with one column); one elsewhere (two columns
+// measured 2048^2 FP32 28.7 against 29.0 us, FP32 1024 13.0 against 12.1, FP64
+// 1024 23.1 against 22.0, FP64 4096 341 against 311; four columns at 2048^2
+// FP32 31.5 us).
+template <typename T, int N> constexpr int p2_teams() { return (sizeof(T) == 8 && N == 2048) ? 2 : 1; }
 
 // Kernel variants chosen per context at run time (shrt_autotune): one code per
 // pass.  Defaults: the fastest at 2048^2 FP32 (P1 31.3 us row against 32.4 us
@@ -143,43 +147,58 @@ struct StepArgs {
 };
 
 // Wavefront max of a double.
-__device__ __forceinline__ double wave_max(double v) {
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
     return v;
 }
 
-// Reduce P2's per-workgroup CFL maxima over the whole grid (every P3
-// workgroup does this; `red` is workgroup-shared scratch of 2*16 doubles).
-__device__ __forceinline__ void cfl_maxima(const double* __restrict__ cflp, int n, double* red, double* cvx,
-                                           double* cvy) {
-    // the pairs as 16-byte loads, eight in flight per thread
-    const double2* p = reinterpret_cast<const double2*>(cflp);
-    double m0 = 0.0, m1 = 0.0;
-    int i = threadIdx.x;
-    for (; i + 7 * (int)blockDim.x < n; i += 8 * blockDim.x) {
-        double2 v[8];
+// P2's per-workgroup CFL maxima, one (max|vx + sigma*y|, max|vy|) pair per
+// P2 workgroup in the working precision (the maxima of working-precision
+// values, so exact), reduced over the whole grid by every P3 workgroup of NT
+// threads.  issue() starts the loads, K per thread (indices past the end load
+// the last pair again, which leaves the maxima unchanged); finish() reduces
+// them over the workgroup (`red` is workgroup-shared scratch of 2*16 doubles).
+// Loads issued between the two travel together with these, and since
+// vector-memory loads complete in order, finish() waits only for the partials.
+template <typename T, int K, int NT>
+struct CflPartials {
+    cplx<T> v[K];
+
+    __device__ __forceinline__ void issue(const double* __restrict__ cflp, int n) {
+        const cplx<T>* p = reinterpret_cast<const cplx<T>*>(cflp);
 #pragma unroll
-        for (int u = 0; u < 8; ++u) v[u] = p[i + u * blockDim.x];
+        for (int u = 0; u < K; ++u) v[u] = p[min((int)threadIdx.x + u * NT, n - 1)];
+    }
+
+    __device__ __forceinline__ void finish(const double* __restrict__ cflp, int n, double* red, double* cvx,
+                                           double* cvy) const {
+        T m0 = 0, m1 = 0;
 #pragma unroll
-        for (int u = 0; u < 8; ++u) { m0 = fmax(m0, v[u].x); m1 = fmax(m1, v[u].y); }
+        for (int u = 0; u < K; ++u) {
+            m0 = fmax(m0, v[u].x);
+            m1 = fmax(m1, v[u].y);
+        }
+        // partials beyond K per thread (none when n <= K * NT)
+        const cplx<T>* p = reinterpret_cast<const cplx<T>*>(cflp);
+        for (int i = threadIdx.x + K * NT; i < n; i += NT) {
+            m0 = fmax(m0, p[i].x);
+            m1 = fmax(m1, p[i].y);
+        }
+        m0 = wave_max(m0);
+        m1 = wave_max(m1);
+        constexpr int nw = (NT + 63) / 64;
+        const int w = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 0) { red[w] = m0; red[16 + w] = m1; }
+        __syncthreads();
+        double r0 = 0.0, r1 = 0.0;
+#pragma unroll
+        for (int i = 0; i < nw; ++i) { r0 = fmax(r0, red[i]); r1 = fmax(r1, red[16 + i]); }
+        *cvx = r0;
+        *cvy = r1;
     }
-    for (; i < n; i += blockDim.x) {
-        const double2 v = p[i];
-        m0 = fmax(m0, v.x);
-        m1 = fmax(m1, v.y);
-    }
-    m0 = wave_max(m0);
-    m1 = wave_max(m1);
-    const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
-    if ((threadIdx.x & 63) == 0) { red[w] = m0; red[16 + w] = m1; }
-    __syncthreads();
-    m0 = 0.0;
-    m1 = 0.0;
-    for (int i = 0; i < nw; ++i) { m0 = fmax(m0, red[i]); m1 = fmax(m1, red[16 + i]); }
-    *cvx = m0;
-    *cvy = m1;
-}
+};
 
 // Ring slots after the renew_fields rotation of step n (time_integral.cu:474-484):
 // a0 (written) = (-(n+1)) mod 3, a1 (state n) = (-n) mod 3, a2 = (1-n) mod 3.

@@ -290,7 +290,7 @@ k_p1_row(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
     }
 }
 
-// The y-inverse transforms of outputs [o_begin, o_end) of the row pair
+// The y-inverse transforms of the outputs in the bit set omask of the row pair
 // (row, -row) from its spectrum before multipliers: base[m] for element
 // e = tid + m*TT (row kx at iky = e, or row -kx at iky = NY - e, not yet
 // conjugated; zero in the padding), base_n0 = the mirror row's ky = 0 value,
@@ -303,7 +303,7 @@ k_p1_row(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
 // kx (2, 4) -- p1_mult in selected form.
 template <typename T, int NY>
 __device__ __forceinline__ void p1_outputs(const StepArgs<T>& a, int row, const cplx<T>* base, cplx<T> base_n0,
-                                           const double* sn, const double* kxr, int o_begin, int o_end,
+                                           const double* sn, const double* kxr, unsigned omask,
                                            int store_phi, cplx<T>* lds, int tid) {
     using S = TeamShape<T, NY>;
     constexpr int P = S::P, TT = S::TT;
@@ -313,7 +313,8 @@ __device__ __forceinline__ void p1_outputs(const StepArgs<T>& a, int row, const 
     const int gp = g_pos_of(row, g.nkxh);
     const TeamSync<S::TT> sync{};
 #pragma nounroll
-    for (int oi = o_begin; oi < o_end; ++oi) {
+    for (unsigned mk = omask; mk != 0; mk &= mk - 1) {
+        const int oi = __builtin_ctz(mk);
         int tl = tid;
         asm volatile("" : "+v"(tl));
         tl &= TT - 1;
@@ -371,7 +372,13 @@ __device__ __forceinline__ void p1_outputs(const StepArgs<T>& a, int row, const 
 // P1 with one (row pair, field group) per workgroup: the row pair of omega
 // (groups 0: vx, vy and 1: d/dx, d/dy omega) or rho (group 2) is read once
 // through the remap view and both outputs of the group are transformed from
-// it; 3 * nkxh workgroups (nkxh in the linear mode, group 0 only).
+// it.  Row 0 (kx = 0) has three transformed outputs, vx, d/dy omega and d/dy
+// rho (the others carry the factor kx = 0 and are written as zeros): its
+// omega workgroup transforms vx and d/dy omega, its rho workgroup d/dy rho and
+// writes the zeros.  The grid is 3 (nkxh - 1) + 2 workgroups, 2048 at 2048^2:
+// all resident at once at eight per CU (3 nkxh = 2049 left one workgroup to
+// start after the first ones finished).  Linear mode: group 0 only, nkxh
+// workgroups.
 template <typename T, int NY, bool BATCH>
 __global__ void __launch_bounds__((TeamShape<T, NY>::TT), (waves_per_eu<T, NY, TeamShape<T, NY>::TT, 1>()))
 k_p1_pair(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
@@ -384,7 +391,11 @@ k_p1_pair(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
     const int tid = threadIdx.x;
     const int ngrp = a.ph.linear ? 1 : 3;
     const int q = xcd_item(blockIdx.x, gridDim.x);
-    const int pos = q / ngrp, grp = q - pos * ngrp;
+    const int nreg = ngrp * (g.nkxh - 1);                 // items of rows 1 .. nkxh-1
+    const bool r0 = !a.ph.linear && q >= nreg;            // row 0: omega (q = nreg) or rho (nreg + 1)
+    const int pos = r0 ? g.nkxh - 1 : q / ngrp;
+    const int grp = r0 ? 2 * (q - nreg) : q - pos * ngrp;
+    const unsigned omask = r0 ? (grp == 0 ? 0x9u : 0x20u) : (3u << (2 * grp));
     const int row = p1_row_of(pos, g.nkxh);
     const int rn = row == 0 ? 0 : g.nkx - row;
     const StepCtl* c = a.ctl_in;
@@ -404,7 +415,15 @@ k_p1_pair(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
         base[m] = keep ? F : czero<T>();
     }
     const cplx<T> base_n0 = view_load(src, rn, 0, g, jmp);
-    p1_outputs<T, NY>(a, row, base, base_n0, sn, kxr, 2 * grp, 2 * grp + 2, a.store_phi, sm, tid);
+    p1_outputs<T, NY>(a, row, base, base_n0, sn, kxr, omask, a.store_phi, sm, tid);
+    if (r0 && grp == 2) {
+        // outputs of row 0 that carry a factor kx = 0: vy, d/dx omega, d/dx rho
+        for (int idx = tid; idx < 3 * NY; idx += TT) {
+            const int z = idx / NY, y = idx - z * NY;
+            const int o = z == 0 ? 1 : (z == 1 ? 2 : 4);
+            a.G[(long long)o * g.I + ((long long)(y >> 2) * g.nkxh + pos) * 4 + (y & 3)] = czero<T>();
+        }
+    }
 }
 
 // max(m, |x|) in the precision of its operands (NaN x leaves m).
@@ -558,14 +577,14 @@ k_p2(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
         }
     }
 
-    double c0 = wave_max((double)m0), c1 = wave_max((double)m1);
+    T c0 = wave_max(m0), c1 = wave_max(m1);
     {
         const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
         if ((threadIdx.x & 63) == 0) { red[w] = c0; red[16 + w] = c1; }
         __syncthreads();
         if (threadIdx.x == 0) {
-            for (int i = 1; i < nw; ++i) { c0 = fmax(c0, red[i]); c1 = fmax(c1, red[16 + i]); }
-            reinterpret_cast<double2*>(a.cflp)[q] = double2{c0, c1};
+            for (int i = 1; i < nw; ++i) { c0 = fmax(c0, (T)red[i]); c1 = fmax(c1, (T)red[16 + i]); }
+            reinterpret_cast<cplx<T>*>(a.cflp)[q] = cplx<T>{c0, c1};
         }
     }
     if (a.ph.linear) return;
@@ -845,8 +864,15 @@ k_p3(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
     const int qq = xcd_item(blockIdx.x, gridDim.x);
     const int myside = SPLIT ? (qq & 1) : 0;   // SPLIT: the row of the pair this workgroup updates
     const int q = SPLIT ? (qq >> 1) : qq;
-    const int tile = q >> 1;
-    const int f = q & 1;   // 0: omega, 1: rho
+    // FUSE: three workgroups per row pair kx >= 1 -- omega with the next step's
+    // vx, vy; omega again with d/dx, d/dy omega (the same update, not stored);
+    // rho with d/dx, d/dy rho -- and two for row 0 (omega: vx, d/dy omega;
+    // rho: d/dy rho and the zero outputs), 3 (nkxh - 1) + 2 in all
+    const int nreg3 = 3 * (g.nkxh - 1);
+    const int fk = FUSE ? (q < nreg3 ? q % 3 : 2 * (q - nreg3)) : 0;
+    const int tile = FUSE ? (q < nreg3 ? q / 3 + 1 : 0) : q >> 1;
+    const int f = FUSE ? (fk == 2 ? 1 : 0) : (q & 1);   // 0: omega, 1: rho
+    const bool writer = !FUSE || fk != 1;               // stores the new state and tendency
     const int row = tile * KT + team;
     const bool act = row < g.nkxh;
     const int rn = row == 0 ? 0 : g.nkx - row;
@@ -950,8 +976,10 @@ k_p3(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
                 an = ab_bdf_update<T>(ist, plain, dt, diff, k2, B.a1[u], B.a2[u], d0, B.d1[u], B.d2[u]);
             }
             const long long at = (long long)r * g.nky + iky;
-            A0[at] = an;
-            D0[at] = d0;
+            if (writer) {
+                A0[at] = an;
+                D0[at] = d0;
+            }
             if constexpr (FUSE) {
                 if (iky == 0) slot[side] = an;
                 else lds[side ? NY - iky : iky] = an;   // the position this thread just read
@@ -961,27 +989,35 @@ k_p3(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
 
     auto run = [&](auto view) {
         Batch bt[2];
+        using FFT = TeamFFT<T, NY, P, S::RMAX, -1>;
+        typename FFT::Twiddles ts;
+        const cplx<T>* Hf = a.H + (long long)f * g.I;
+        cplx<T> v[P];
 
         // the step size after the CFL halvings, and (workgroup 0) the bookkeeping
-        cfl_maxima(a.cflp, a.ncflp, red, &cvx, &cvy);
+        {
+            CflPartials<T, (NY / NT > 16 ? 16 : (NY / NT < 1 ? 1 : NY / NT)), NT> cp;
+            cp.issue(a.cflp, a.ncflp);
+            cp.finish(a.cflp, a.ncflp, red, &cvx, &cvy);
+        }
         const int h = cfl_delt(c, g, ph, cvx, cvy, &dt, &which);
         if (blockIdx.x == 0) p3_bookkeeping(a, dt, h, which, cvx, cvy);
-
-        if (!ph.linear) {
-            // twiddles and the H tile of this field, then [team][y] in LDS
-            using FFT = TeamFFT<T, NY, P, S::RMAX, -1>;
-            typename FFT::Twiddles ts;
-            ts.load(a.twy, tid);
-            const cplx<T>* Hf = a.H + (long long)f * g.I;
-            cplx<T> v[P];
-            if constexpr (KT == 1) {
-                // this row of the [t][kx][c] tiles: runs of 4 (32 bytes) per t
+        if constexpr (KT == 1) {
+            if (!ph.linear) {
+                // twiddles and this row of the [t][kx][c] tiles of H: runs of 4 (32 bytes) per t
+                ts.load(a.twy, tid);
 #pragma unroll
                 for (int m = 0; m < P; ++m) {
                     const int y = tid + m * TT;
                     v[m] = Hf[((long long)(y >> 2) * g.nkxh + row) * 4 + (y & 3)];
                 }
-            } else {
+            }
+        }
+
+        if (!ph.linear) {
+            if constexpr (KT != 1) {
+                // twiddles and the H tile of these rows, then [team][y] in LDS
+                ts.load(a.twy, tid);
                 cplx<T> hv[P];
 #pragma unroll
                 for (int m = 0; m < P; ++m) {   // NY*4 tile elements over NT threads
@@ -1024,6 +1060,7 @@ k_p3(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
 
     if constexpr (FUSE) {
         if (!act || (ph.linear && f == 1)) return;
+        if (ph.linear && fk == 1) return;   // no d/dx, d/dy outputs in the linear mode
         __syncthreads();   // the new rows are in LDS
         // shift and remap of this step for the two rows (p3_bookkeeping's operations)
         const double sdt1 = g.nkx > 1 ? mul_rn(mul_rn(ph.sigma, a.kx[1]), dt) : 0.0;
@@ -1063,8 +1100,18 @@ k_p3(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
         }
         const cplx<T> base_n0 = viewval(1, 0);   // the mirror row's ky = 0 value
         __syncthreads();   // LDS is the transforms' scratch from here on
-        const int o_end = f == 0 ? (ph.linear ? 2 : 4) : 6;
-        p1_outputs<T, NY>(a, row, base, base_n0, sn, kx_r, f == 0 ? 0 : 4, o_end, a.store_phi_next, lds, tid);
+        unsigned omask = row == 0 ? (f == 0 ? 0x9u : 0x20u) : (3u << (2 * fk));
+        if (ph.linear) omask = (row == 0 && f == 0) ? 0x3u : (omask & 0x3u);   // vx, vy only
+        p1_outputs<T, NY>(a, row, base, base_n0, sn, kx_r, omask, a.store_phi_next, lds, tid);
+        if (row == 0 && f == 1 && !ph.linear) {
+            // outputs of row 0 that carry a factor kx = 0: vy, d/dx omega, d/dx rho
+            const int p0 = g.nkxh - 1;
+            for (int idx = tid; idx < 3 * NY; idx += TT) {
+                const int z = idx / NY, y = idx - z * NY;
+                const int o = z == 0 ? 1 : (z == 1 ? 2 : 4);
+                a.G[(long long)o * g.I + ((long long)(y >> 2) * g.nkxh + p0) * 4 + (y & 3)] = czero<T>();
+            }
+        }
     }
 }
 
