@@ -248,12 +248,14 @@ CLOSURE_DEFAULT = dict(n_fine=256, n_coarse=64, n_mid=128, delt=1e-3, interval=2
 # CFL number) filtered to 256^2, 64 trajectories x 4000 steps (1000 of spin-up, then
 # a snapshot every 20 steps), a 4-layer width-32 network on 32 modes per axis, 16
 # held-out seeds; plain 256^2, 512^2, 1024^2 and 2048^2 runs give the
-# error-versus-GPU-time curve the learned 256^2 run is placed on.
+# error-versus-GPU-time curve the learned 256^2 run is placed on.  The correction
+# strength (0 = plain) and band are chosen by 4000-step rollouts on the 4
+# validation trajectories before the held-out seeds are run.
 CLOSURE_CONFIG5 = dict(n_fine=2048, n_coarse=256, n_mid=512, delt=1.25e-4, interval=20, spinup=1000, snaps=150,
                        nu=1e-4, rho0_prime=-1.0, amp=2.0, kmin=1.0, kmax_ic=32.0, kmax_net=32, width=32, layers=4,
                        cfl=0.5, trajectories=64, val_trajectories=4, test_seeds=list(range(900, 916)),
                        eval_steps=4000, eval_grids=[512, 1024, 2048], host_path=False, iters=3000, batch=16,
-                       kcut=32, guard=0.1)
+                       kcut=32, guard=0.1, calib_alphas=[0.125, 0.25, 0.5, 1.0], calib_kcuts=[8, 16, 32])
 
 
 def run_closure(shrt, args):
@@ -269,6 +271,7 @@ def run_closure(shrt, args):
     iters, lr, batch = kw.pop("iters", 2000), kw.pop("lr", 5e-4), kw.pop("batch", 16)
     tests, evsteps = kw.pop("test_seeds", [900, 901, 902, 903]), kw.pop("eval_steps", 1000)
     grids, host_path = kw.pop("eval_grids", None), kw.pop("host_path", True)
+    alphas, kcuts = kw.pop("calib_alphas", None), kw.pop("calib_kcuts", None)
     cfg = cl.Config(**kw)
     p = cl.Pipeline(shrt, cfg)
     t0 = time.perf_counter()
@@ -283,6 +286,12 @@ def run_closure(shrt, args):
     t_train = time.perf_counter() - t0
     del x, y
     log(f"closure: trained {iters} iterations in {t_train:.1f} s, best validation R2 {p.val_best:.3f}")
+    calib = None
+    if alphas:
+        # correction strength and band from rollouts on the validation seeds (alpha 0: plain)
+        t0 = time.perf_counter()
+        calib = p.calibrate(range(800, 800 + nval), evsteps, alphas=alphas, kcuts=kcuts, log=log)
+        log(f"closure: calibrated alpha {cfg.alpha}, kcut {cfg.kcut} in {time.perf_counter() - t0:.1f} s")
     res = p.evaluate(tests, evsteps, grids=grids, host_path=host_path, log=log)
     runs = list(res[0]["wall_s"].keys())
     mean = lambda run, key: float(np.mean([r[run][key] for r in res]))
@@ -292,6 +301,7 @@ def run_closure(shrt, args):
                           eval_grids=grids),
            "samples": int(p.nsamples), "data_s": t_data, "train_s": t_train, "train_loss_last": float(losses[-1]),
            "val_r2_best": p.val_best, "val_curve": p.val_curve,
+           "calibration": calib, "alpha": cfg.alpha, "kcut": cfg.kcut,
            "error": {run: {"enstrophy_spectrum": mean(run, "enstrophy_spectrum"), "rho": mean(run, "rho")}
                      for run in runs},
            "wall_s_per_run": {run: wall(run) for run in runs},

@@ -12,9 +12,15 @@ tendency defect (F_{j+1} - C_j) / (interval * delt) of omega and rho, and the
 inputs are the physical omega, rho, phi of F_j plus the shear rate.
 
 Use.  Every `interval` steps the corrected coarse run adds
-interval * delt * net(state) to omega and rho.  The correction is skipped
-(fallback to the plain coarse step) when the network output is not finite or
-larger than `guard` times the field it corrects.
+alpha * interval * delt * net(state) to omega and rho, on the modes with
+|kx|, ky <= kcut.  The correction is skipped (fallback to the plain coarse
+step) when the network output is not finite or larger than `guard` times the
+field it corrects.  The strength alpha and the band kcut are chosen after
+training by rollouts on the validation trajectories (`calibrate`): the network
+is fitted to one-interval defects, and how much of its prediction helps over
+thousands of steps is a property of the rollout, not of the fit; alpha = 0 is
+among the candidates, so the calibrated run falls back to the plain solver
+when no correction helps on the validation seeds.
 
 Error measure.  Against the filtered fine run: rho relative L2 on the coarse
 modes and the relative L2 of the shell-summed enstrophy spectrum
@@ -48,6 +54,7 @@ class Config:
         self.kmax_net, self.width, self.layers = 10, 32, 4
         self.guard = 0.5
         self.kcut = 0                # > 0: corrections kept to |kx|, ky <= kcut
+        self.alpha = 1.0             # strength of the applied correction (0: the plain coarse solver)
         for k, v in kw.items():
             if not hasattr(self, k):
                 raise KeyError(k)
@@ -263,6 +270,7 @@ class Pipeline:
             shn[rows] = sh
             self.load_state(c, Om, Rh, t, shn)
         skipped = 0
+        closure = closure and cfg.alpha > 0   # strength 0: the plain solver
         c.synchronize()
         t0 = time.perf_counter()
         done = 0
@@ -272,13 +280,13 @@ class Pipeline:
         while done < steps:
             k = min(cfg.interval, steps - done)
             if closure and device and k == cfg.interval:
-                if not self.net.correct(c, self.scale_in, sigma, self.scale_out, cfg.interval * cfg.delt, cfg.guard,
-                                        cfg.kcut):
+                if not self.net.correct(c, self.scale_in, sigma, self.scale_out, cfg.alpha * cfg.interval * cfg.delt,
+                                        cfg.guard, cfg.kcut):
                     skipped += 1
             elif closure and k == cfg.interval:
                 corr = self.correction(c, sigma)
                 om_c, rh_c = c.spectral(S.OMEGA), c.spectral(S.RHO)
-                dt_tot = cfg.interval * cfg.delt
+                dt_tot = cfg.alpha * cfg.interval * cfg.delt
                 if corr is None or (np.linalg.norm(corr[0]) * dt_tot > cfg.guard * np.linalg.norm(om_c) or
                                     np.linalg.norm(corr[1]) * dt_tot > cfg.guard * np.linalg.norm(rh_c)):
                     skipped += 1
@@ -295,6 +303,39 @@ class Pipeline:
             out = (f(c.spectral(S.OMEGA)), f(c.spectral(S.RHO)))
         c.close()
         return out, wall, skipped
+
+    def calibrate(self, seeds, steps, alphas=(0.0, 0.25, 0.5, 1.0), kcuts=None, log=None):
+        """Choose the correction strength alpha and band kcut by rollouts on
+        (validation) seeds: every candidate runs `steps` corrected coarse steps
+        from each seed's first filtered snapshot, and the pair with the lowest
+        mean enstrophy-spectrum error against the filtered fine run is kept in
+        the configuration (alpha = 0 is the plain coarse solver).  Returns the
+        table of mean errors per candidate."""
+        cfg = self.cfg
+        kcuts = [cfg.kcut] if kcuts is None else list(kcuts)
+        cands = [(0.0, cfg.kcut)] + [(a, k) for a in alphas if a > 0 for k in kcuts]
+        starts = []
+        for seed in seeds:
+            snaps = self.fine_snapshots(seed, steps // cfg.interval + 1)
+            starts.append((seed, cfg.sigma_of(seed), snaps[0], snaps[-1][:2]))
+        saved = (cfg.alpha, cfg.kcut)
+        table = []
+        for a, k in cands:
+            cfg.alpha, cfg.kcut = a, k
+            errs = []
+            for seed, sigma, (om, rh, t, sh, _), ref in starts:
+                out, _, _ = self.run_coarse(om, rh, t, sh, sigma, steps, closure=True, device=True)
+                errs.append(self.errors(out, ref))
+            row = dict(alpha=a, kcut=k, enstrophy_spectrum=float(np.mean([e["enstrophy_spectrum"] for e in errs])),
+                       rho=float(np.mean([e["rho"] for e in errs])))
+            table.append(row)
+            if log is not None:
+                log(f"closure calibration: alpha {a} kcut {k}: enstrophy spectrum {row['enstrophy_spectrum']:.4f}, "
+                    f"rho {row['rho']:.4f}")
+        cfg.alpha, cfg.kcut = saved
+        best = min(table, key=lambda r: (r["enstrophy_spectrum"], r["alpha"]))
+        cfg.alpha, cfg.kcut = best["alpha"], best["kcut"]
+        return table
 
     def errors(self, run, ref):
         kx, ky = k_ints(self.sc)

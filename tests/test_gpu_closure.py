@@ -226,9 +226,11 @@ def test_time_to_error_pipeline(gpu):
     config5) at reduced size: 256^2 fine runs filtered on the device to 64^2,
     plain 64^2, 128^2 and 256^2 runs and the learned 64^2 run from the same
     filtered states on two held-out seeds, each timed, and the plain method's
-    time at the learned run's error.  Checks the measurement itself: finite
-    errors, the finest plain grid closest to the filtered fine run, and a
-    time ratio bracketed by the plain curve."""
+    time at the learned run's error, after the correction strength and band
+    are calibrated on a validation seed.  Checks the measurement itself: the
+    calibration keeps its best candidate, finite errors, the finest plain grid
+    closest to the filtered fine run, the fallback equal to the plain run when
+    the calibration chose it, and a time ratio bracketed by the plain curve."""
     import importlib.util
     here = os.path.dirname(os.path.abspath(__file__))
     spec = importlib.util.spec_from_file_location("closure_pipe", os.path.join(here, "..", "shearing-rt_amd",
@@ -242,12 +244,21 @@ def test_time_to_error_pipeline(gpu):
     p = cl.Pipeline(shrt, cfg)
     x, y = p.generate(range(100, 108))
     p.fit(x, y, iters=200, batch=8, lr=5e-4)
+    # correction strength and band from rollouts on a validation seed; alpha 0 is the plain solver
+    table = p.calibrate([800], 400, alphas=(0.5, 1.0), kcuts=[4, 8])
+    assert len(table) == 5 and table[0]["alpha"] == 0.0
+    assert all(np.isfinite(t["enstrophy_spectrum"]) for t in table)
+    best = min(table, key=lambda t: (t["enstrophy_spectrum"], t["alpha"]))
+    assert (cfg.alpha, cfg.kcut) == (best["alpha"], best["kcut"])
+    print("  calibration", [(t["alpha"], t["kcut"], round(t["enstrophy_spectrum"], 4)) for t in table], flush=True)
     res = p.evaluate([900, 901], 400, grids=[128, 256], host_path=False)
     runs = ["plain", "plain_128", "plain_256", "closure_device"]
     for r in res:
         for k in runs:
             assert np.isfinite(r[k]["enstrophy_spectrum"]) and np.isfinite(r[k]["rho"]) and r["wall_s"][k] > 0, k
         assert r["plain_256"]["enstrophy_spectrum"] < r["plain"]["enstrophy_spectrum"]
+        if cfg.alpha == 0:   # the fallback: the learned run is the plain coarse run
+            assert r["closure_device"] == r["plain"]
         print("  seed", r["seed"], {k: round(r[k]["enstrophy_spectrum"], 4) for k in runs},
               {k: round(r["wall_s"][k] * 1e3, 2) for k in runs})
     mean = lambda k: float(np.mean([r[k]["enstrophy_spectrum"] for r in res]))
