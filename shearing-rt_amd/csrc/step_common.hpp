@@ -154,51 +154,39 @@ __device__ __forceinline__ T wave_max(T v) {
     return v;
 }
 
-// P2's per-workgroup CFL maxima, one (max|vx + sigma*y|, max|vy|) pair per
-// P2 workgroup in the working precision (the maxima of working-precision
-// values, so exact), reduced over the whole grid by every P3 workgroup of NT
-// threads.  issue() starts the loads, K per thread (indices past the end load
-// the last pair again, which leaves the maxima unchanged); finish() reduces
-// them over the workgroup (`red` is workgroup-shared scratch of 2*16 doubles).
-// Loads issued between the two travel together with these, and since
-// vector-memory loads complete in order, finish() waits only for the partials.
-template <typename T, int K, int NT>
-struct CflPartials {
-    cplx<T> v[K];
-
-    __device__ __forceinline__ void issue(const double* __restrict__ cflp, int n) {
-        const cplx<T>* p = reinterpret_cast<const cplx<T>*>(cflp);
+// Reduce P2's per-workgroup CFL maxima over the whole grid (every P3
+// workgroup does this; `red` is workgroup-shared scratch of 2*16 doubles).
+// One (max|vx + sigma*y|, max|vy|) pair per P2 workgroup in the working
+// precision (the maxima of working-precision values, so exact), eight loads
+// in flight per thread.
+template <typename T>
+__device__ __forceinline__ void cfl_maxima(const double* __restrict__ cflp, int n, double* red, double* cvx,
+                                           double* cvy) {
+    const cplx<T>* p = reinterpret_cast<const cplx<T>*>(cflp);
+    T m0 = 0, m1 = 0;
+    int i = threadIdx.x;
+    for (; i + 7 * (int)blockDim.x < n; i += 8 * blockDim.x) {
+        cplx<T> v[8];
 #pragma unroll
-        for (int u = 0; u < K; ++u) v[u] = p[min((int)threadIdx.x + u * NT, n - 1)];
+        for (int u = 0; u < 8; ++u) v[u] = p[i + u * blockDim.x];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) { m0 = fmax(m0, v[u].x); m1 = fmax(m1, v[u].y); }
     }
-
-    __device__ __forceinline__ void finish(const double* __restrict__ cflp, int n, double* red, double* cvx,
-                                           double* cvy) const {
-        T m0 = 0, m1 = 0;
-#pragma unroll
-        for (int u = 0; u < K; ++u) {
-            m0 = fmax(m0, v[u].x);
-            m1 = fmax(m1, v[u].y);
-        }
-        // partials beyond K per thread (none when n <= K * NT)
-        const cplx<T>* p = reinterpret_cast<const cplx<T>*>(cflp);
-        for (int i = threadIdx.x + K * NT; i < n; i += NT) {
-            m0 = fmax(m0, p[i].x);
-            m1 = fmax(m1, p[i].y);
-        }
-        m0 = wave_max(m0);
-        m1 = wave_max(m1);
-        constexpr int nw = (NT + 63) / 64;
-        const int w = threadIdx.x >> 6;
-        if ((threadIdx.x & 63) == 0) { red[w] = m0; red[16 + w] = m1; }
-        __syncthreads();
-        double r0 = 0.0, r1 = 0.0;
-#pragma unroll
-        for (int i = 0; i < nw; ++i) { r0 = fmax(r0, red[i]); r1 = fmax(r1, red[16 + i]); }
-        *cvx = r0;
-        *cvy = r1;
+    for (; i < n; i += blockDim.x) {
+        const cplx<T> v = p[i];
+        m0 = fmax(m0, v.x);
+        m1 = fmax(m1, v.y);
     }
-};
+    m0 = wave_max(m0);
+    m1 = wave_max(m1);
+    const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    if ((threadIdx.x & 63) == 0) { red[w] = m0; red[16 + w] = m1; }
+    __syncthreads();
+    double r0 = 0.0, r1 = 0.0;
+    for (int i = 0; i < nw; ++i) { r0 = fmax(r0, red[i]); r1 = fmax(r1, red[16 + i]); }
+    *cvx = r0;
+    *cvy = r1;
+}
 
 // Ring slots after the renew_fields rotation of step n (time_integral.cu:474-484):
 // a0 (written) = (-(n+1)) mod 3, a1 (state n) = (-n) mod 3, a2 = (1-n) mod 3.

@@ -989,35 +989,27 @@ k_p3(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
 
     auto run = [&](auto view) {
         Batch bt[2];
-        using FFT = TeamFFT<T, NY, P, S::RMAX, -1>;
-        typename FFT::Twiddles ts;
-        const cplx<T>* Hf = a.H + (long long)f * g.I;
-        cplx<T> v[P];
 
         // the step size after the CFL halvings, and (workgroup 0) the bookkeeping
-        {
-            CflPartials<T, (NY / NT > 16 ? 16 : (NY / NT < 1 ? 1 : NY / NT)), NT> cp;
-            cp.issue(a.cflp, a.ncflp);
-            cp.finish(a.cflp, a.ncflp, red, &cvx, &cvy);
-        }
+        cfl_maxima<T>(a.cflp, a.ncflp, red, &cvx, &cvy);
         const int h = cfl_delt(c, g, ph, cvx, cvy, &dt, &which);
         if (blockIdx.x == 0) p3_bookkeeping(a, dt, h, which, cvx, cvy);
-        if constexpr (KT == 1) {
-            if (!ph.linear) {
-                // twiddles and this row of the [t][kx][c] tiles of H: runs of 4 (32 bytes) per t
-                ts.load(a.twy, tid);
+
+        if (!ph.linear) {
+            // twiddles and the H tile of this field, then [team][y] in LDS
+            using FFT = TeamFFT<T, NY, P, S::RMAX, -1>;
+            typename FFT::Twiddles ts;
+            ts.load(a.twy, tid);
+            const cplx<T>* Hf = a.H + (long long)f * g.I;
+            cplx<T> v[P];
+            if constexpr (KT == 1) {
+                // this row of the [t][kx][c] tiles: runs of 4 (32 bytes) per t
 #pragma unroll
                 for (int m = 0; m < P; ++m) {
                     const int y = tid + m * TT;
                     v[m] = Hf[((long long)(y >> 2) * g.nkxh + row) * 4 + (y & 3)];
                 }
-            }
-        }
-
-        if (!ph.linear) {
-            if constexpr (KT != 1) {
-                // twiddles and the H tile of these rows, then [team][y] in LDS
-                ts.load(a.twy, tid);
+            } else {
                 cplx<T> hv[P];
 #pragma unroll
                 for (int m = 0; m < P; ++m) {   // NY*4 tile elements over NT threads
