@@ -362,8 +362,9 @@ def main():
     ap.add_argument("--kernel-steps", type=int, default=50)
     ap.add_argument("--variants", default="", metavar="P1,P2,P3",
                     help="kernel variants of the three passes (shrt_set_variants); default: the library's")
-    ap.add_argument("--fusion", type=int, choices=[0, 1], default=0,
-                    help="captured batches fuse P3 of step n with P1 of step n+1 (1) or not (0, default)")
+    ap.add_argument("--fusion", type=int, choices=[0, 1], default=None,
+                    help="captured batches fuse P3 of step n with P1 of step n+1 (1) or not (0); "
+                         "default: the library's choice for the size and precision")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU-baseline work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="BYTES",
@@ -452,7 +453,8 @@ def main():
 
     log(f"bench: {n}^2 {args.precision}, graph_steps={args.graph_steps}, warmup={args.warmup}, steps={args.steps}")
     g = shrt.Shrt(nx=n, ny=n, precision=prec, graph_steps=args.graph_steps, **params)
-    g.set_fusion(args.fusion)
+    if args.fusion is not None:
+        g.set_fusion(args.fusion)
     if args.variants:
         g.set_variants(*[int(v) for v in args.variants.split(",")])
     used = g.variants()
@@ -500,7 +502,7 @@ def main():
         "data": "synthetic: omega=0, seeded random-phase rho on 2<=|k|<=32, amplitude 1e-4|k|^-1.5",
         "config": {"workload": f"pseudospectral step, {n}^2 {args.precision.upper()} single instance "
                                "(sigma=1, nu=kappa=1e-4, g=1, delt=1e-4)",
-                   "nx": n, "ny": n, "graph_steps": args.graph_steps, "fusion": args.fusion,
+                   "nx": n, "ny": n, "graph_steps": args.graph_steps, "fusion": used["fuse"],
                    "variants": f"{used['p1']},{used['p2']},{used['p3']}" + ("" if args.variants else " (default)")},
         "roofline": {"bound": "hbm", "kernel": dname, "achieved": achieved, "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,

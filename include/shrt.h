@@ -243,8 +243,9 @@ shrt_status shrt_stream_copy_bench(long long bytes, int reps, double* ms);
  * p1 pair, p2 three, p2 four, p3 pair, p3 tile, p3 split.  It then times
  * captured batches of graph_steps steps (20 if the context has none) with and
  * without the P3+P1 fusion (shrt_set_fusion) and keeps the faster.  Without
- * autotuning a context starts with P1 variant 2 for ny >= 2048, 1 for ny = 512
- * and for ny = 1024 in FP32, 0 otherwise (P2 0, P3 0, no fusion): the picks of
+ * autotuning a context starts with P1 variant 2 for ny = 2048 in FP32 and for
+ * ny >= 4096, 1 for ny = 512, 0 otherwise (P2 0, P3 0), and with the P3+P1
+ * fusion on for FP32 at ny = 512 and 1024, off otherwise: the picks of
  * shrt_autotune on MI355X.  shrt_get_variants reports the current choice. */
 typedef struct {
     int p1, p2, p3;
@@ -272,10 +273,11 @@ shrt_status shrt_set_update_precision(shrt_ctx* ctx, int wide);
  * of graph_steps steps runs P3 of step n and P1 of step n+1 as one kernel: the
  * updated rows stay in LDS, the next shift and remap are formed in the kernel,
  * and the next step's y-inverse transforms are written without reading the new
- * state back.  on = 0 (the default) runs the three passes per step.  Single
- * steps (shrt_step with fewer than graph_steps left) never fuse.  Fusion saves
- * P1's reads but gives a row pair's workgroup its transforms in series, so it
- * pays on some shapes only; shrt_autotune times both and keeps the faster. */
+ * state back.  on = 0 runs the three passes per step; a context starts with
+ * fusion on for FP32 at ny = 512 and 1024, off otherwise.  Single steps
+ * (shrt_step with fewer than graph_steps left) never fuse.  Fusion saves P1's
+ * reads and one launch per step but pays on some shapes only; shrt_autotune
+ * times both and keeps the faster. */
 shrt_status shrt_set_fusion(shrt_ctx* ctx, int on);
 
 /* Ensembles (new; no reference counterpart: the reference runs one instance

@@ -570,6 +570,10 @@ shrt_status shrt_create(const shrt_params* p, shrt_ctx** out) {
     // of 1024 and of FP64 2048 fill the chip with one output per workgroup).
     if (g.ny >= 4096 || (g.ny == 2048 && c->prec == 4)) c->var.p1 = kP1Pair;
     else if (g.ny == 512) c->var.p1 = kP1Tile;
+    // Captured batches fuse each step's P3 with the next step's P1 by default
+    // where that measured faster on MI355X: FP32 at 512 and 1024 (1024^2 35.2
+    // against 37.1 us per step); equal at 2048, slower at 256, at 4096 and in FP64.
+    if (c->prec == 4 && (g.ny == 512 || g.ny == 1024)) c->var.fuse = 1;
 
     auto bail = [&](shrt_status st) {
         shrt_destroy(c);
