@@ -218,6 +218,12 @@ shrt_status shrt_get_halvings(shrt_ctx* ctx, shrt_halving* out, int cap, int* co
  * pass). */
 shrt_status shrt_time_steps(shrt_ctx* ctx, int nsteps, double* ms_total);
 shrt_status shrt_kernel_times(shrt_ctx* ctx, int nsteps, double* ms3);
+/* The fused kernel's duration (no reference counterpart): `nsteps` + 1 steps
+ * launched as a fused captured batch launches them (P1; P2 and the fused P3 of
+ * a step with P1 of the next, per step; P2 and P3 last), each fused dispatch
+ * recording events at its own start and end; *ms = its average duration in
+ * milliseconds.  Refused for the FP32-arithmetic update, which runs unfused. */
+shrt_status shrt_fused_kernel_time(shrt_ctx* ctx, int nsteps, double* ms);
 
 /* Benchmark support (no reference counterpart): copy `bytes` (a multiple of 16)
  * between two fresh device buffers with a plain copy kernel, `reps` times, and
@@ -245,7 +251,7 @@ shrt_status shrt_stream_copy_bench(long long bytes, int reps, double* ms);
  * without the P3+P1 fusion (shrt_set_fusion) and keeps the faster.  Without
  * autotuning a context starts with P1 variant 2 for ny = 2048 in FP32 and for
  * ny >= 4096, 1 for ny = 512, 0 otherwise (P2 0, P3 0), and with the P3+P1
- * fusion on for FP32 at ny = 512 and 1024, off otherwise: the picks of
+ * fusion on for FP32 at ny = 512, 1024 and 2048, off otherwise: the picks of
  * shrt_autotune on MI355X.  shrt_get_variants reports the current choice. */
 typedef struct {
     int p1, p2, p3;
@@ -274,7 +280,7 @@ shrt_status shrt_set_update_precision(shrt_ctx* ctx, int wide);
  * updated rows stay in LDS, the next shift and remap are formed in the kernel,
  * and the next step's y-inverse transforms are written without reading the new
  * state back.  on = 0 runs the three passes per step; a context starts with
- * fusion on for FP32 at ny = 512 and 1024, off otherwise.  Single steps
+ * fusion on for FP32 at ny = 512, 1024 and 2048, off otherwise.  Single steps
  * (shrt_step with fewer than graph_steps left) never fuse.  Fusion saves P1's
  * reads and one launch per step but pays on some shapes only; shrt_autotune
  * times both and keeps the faster. */

@@ -125,6 +125,7 @@ _SIGS = {
                           ctypes.c_int),
     "shrt_time_steps": ([_VP, ctypes.c_int, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "shrt_kernel_times": ([_VP, ctypes.c_int, _VP], ctypes.c_int),
+    "shrt_fused_kernel_time": ([_VP, ctypes.c_int, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "shrt_copy_bench": ([ctypes.c_longlong, ctypes.c_int, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "shrt_stream_copy_bench": ([ctypes.c_longlong, ctypes.c_int, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "shrt_sizes": ([ctypes.c_int, ctypes.c_int, _VP], ctypes.c_int),
@@ -321,6 +322,12 @@ class Shrt:
         out = np.zeros(3)
         _check(lib().shrt_kernel_times(self.h, int(n), _ptr(out)))
         return out
+
+    def fused_kernel_time(self, n):
+        """Average milliseconds per launch of the fused P3+P1 kernel over n launches."""
+        ms = ctypes.c_double()
+        _check(lib().shrt_fused_kernel_time(self.h, int(n), ctypes.byref(ms)))
+        return ms.value
 
     def autotune(self, reps=20):
         t = Tuning()

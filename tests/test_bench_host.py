@@ -52,6 +52,24 @@ def test_pmc_summary(tmp_path):
     assert set(out) == {"calibration", "p1_2048_fp32", "p2_2048_fp32", "p3_1024_fp64"}
 
 
+def test_fused_kernel_bytes_and_counter_key(tmp_path):
+    # P3 of step n with P1 of step n+1: P3's bytes plus P1's writes, no state read
+    s = {"M": 1365 * 683, "I": 683 * 2048}
+    kb = bench.kernel_bytes(s, 8)
+    assert bench.fused_bytes(s, 8) == kb["p3"] + 8 * 6 * s["I"] == kb["p3"] + kb["p1"] - 8 * 2 * s["M"]
+    cal = 1 << 20
+    _csv(tmp_path / "cf.csv", [("void shrt::k_copy16<float>(x)", cal / 2 / 1024)])
+    _csv(tmp_path / "cw.csv", [("void shrt::k_copy16<float>(x)", cal / 1024)])
+    p3 = "void shrt::k_p3<float, 2048, false, 1, false, false, false>(shrt::StepArgs<float>, a)"
+    p31 = "void shrt::k_p3<float, 2048, true, 1, true, false, false>(shrt::StepArgs<float>, a)"
+    _csv(tmp_path / "f.csv", [(p3, 10.0), (p31, 30.0)])
+    _csv(tmp_path / "w.csv", [(p3, 5.0), (p31, 20.0)])
+    out = bench.pmc_summary(str(tmp_path / "f.csv"), str(tmp_path / "w.csv"), str(tmp_path / "cf.csv"),
+                            str(tmp_path / "cw.csv"), cal)
+    assert out["p3_2048_fp32"]["kernel"] == p3 and out["p31_2048_fp32"]["kernel"] == p31
+    assert out["p31_2048_fp32"]["bytes"] == pytest.approx((2 * 30.0 + 20.0) * 1024)
+
+
 def test_pmc_summary_needs_the_copy(tmp_path):
     _csv(tmp_path / "x.csv", [("void shrt::k_p2<float, 2048, false, true>(a)", 1.0)])
     with pytest.raises(RuntimeError):

@@ -759,6 +759,31 @@ def test_kernel_variants(gpu, prec):
     g.close()
 
 
+@pytest.mark.parametrize("n,prec", [(128, 8), (128, 4), (2048, 4)])
+def test_fused_kernel_time_matches_a_fused_batch(gpu, n, prec):
+    """shrt_fused_kernel_time launches nsteps + 1 steps exactly as a fused
+    captured batch of that many steps does (at 2048^2 FP32 with the fused
+    kernel's arguments in device memory): the state after it is the batch's,
+    bit for bit, and the timing is positive."""
+    print(f"fused kernel timing n={n} prec={prec}", flush=True)
+    ic = dict(seed=17, kmin=2.0, kmax=8.0, amp=1e-3, expo=-1.0)
+    g = shrt.Shrt(nx=n, ny=n, precision=prec, graph_steps=6, **dict(BASE, sigma=1.0))
+    g.set_fusion(True)
+    g.initialize_seeded(**ic)
+    g.time_advance(6)
+    ref = (g.spectral(shrt.OMEGA), g.spectral(shrt.RHO), g.spectral(shrt.PHI), g.state())
+    g.close()
+    g = shrt.Shrt(nx=n, ny=n, precision=prec, graph_steps=0, **dict(BASE, sigma=1.0))
+    g.initialize_seeded(**ic)
+    ms = g.fused_kernel_time(5)
+    out = (g.spectral(shrt.OMEGA), g.spectral(shrt.RHO), g.spectral(shrt.PHI), g.state())
+    g.close()
+    assert ms > 0
+    for a, b in zip(out[:3], ref[:3]):
+        assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
+    assert out[3] == ref[3] and out[3]["istep"] == 6
+
+
 def test_copy_benches(gpu):
     """The bandwidth references of the bench: both copy kernels time a copy and
     reject sizes that are not whole 16-byte words."""
