@@ -507,7 +507,12 @@ def main():
     achieved = kb_all[dname] / (kms[dname] * 1e-3) / 1e9
     traffic, traffic_src = read_traffic(f"{dname}_{n}_{args.precision}")
     ms_step = ms / args.steps
-    step_bytes = sum(kb.values())
+    if kt31 is not None:
+        # a fused batch of G steps: one P1, G P2, G-1 fused P3+P1, one P3
+        G = max(1, min(args.graph_steps, args.steps))
+        step_bytes = kb["p2"] + ((G - 1) * kb_all["p31"] + kb["p1"] + kb["p3"]) / G
+    else:
+        step_bytes = sum(kb.values())
     baseline_formula = c * (14 * s["M"] + 10 * s["I"])
 
     out = {
