@@ -10,7 +10,8 @@
 namespace shrt {
 namespace nn {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 // FP32 -> BF16, round to nearest even (NaN kept quiet).
@@ -36,24 +37,25 @@ struct GemmArgs {
     float alpha, beta;
 };
 
-constexpr int kBM = 64, kBN = 64, kBK = 16, kLdsK = kBK + 4;   // padded rows against bank conflicts
+constexpr int kBM = 64, kBN = 64, kBK = 32, kLdsK = kBK + 8;   // padded rows (80 bytes) against bank conflicts
 
 // 256 threads: four wavefronts, each a 32x32 quarter of the 64x64 tile as 2x2
-// MFMA 16x16x16 blocks.  Per K step every thread stages four consecutive k of
-// one A row and of one B column as BF16 in LDS ([m][k] and [n][k]); a lane's
-// MFMA operand is then one 8-byte LDS read: A[row = lane%16][k = 4*(lane/16)..+3],
-// B[k = 4*(lane/16)..+3][col = lane%16]; the result D[4*(lane/16) + r][lane%16]
+// MFMA 16x16x32 blocks (gfx950's BF16 matrix instruction with K = 32).  Per K
+// step every thread stages eight consecutive k of one A row and of one B
+// column as BF16 in LDS ([m][k] and [n][k]); a lane's MFMA operand is then one
+// 16-byte LDS read: A[row = lane%16][k = 8*(lane/16)..+7],
+// B[k = 8*(lane/16)..+7][col = lane%16]; the result D[4*(lane/16) + r][lane%16]
 // sits in register r.
 __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
-    __shared__ unsigned short As[kBM * kLdsK];
-    __shared__ unsigned short Bs[kBN * kLdsK];
+    __shared__ __attribute__((aligned(16))) unsigned short As[kBM * kLdsK];
+    __shared__ __attribute__((aligned(16))) unsigned short Bs[kBN * kLdsK];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int n0 = blockIdx.x * kBN, m0 = blockIdx.y * kBM;
     const int z = blockIdx.z / g.ksplit, split = blockIdx.z - z * g.ksplit;
     const int kbeg = split * g.kchunk, kend = min(g.K, kbeg + g.kchunk);
     const float* A = g.A + z * g.sAz;
     const float* B = g.B + z * g.sBz;
-    const int lr = t >> 2, lk = (t & 3) * 4;    // staging: row/col lr, k lk..lk+3
+    const int lr = t >> 2, lk = (t & 3) * 8;    // staging: row/col lr, k lk..lk+7
     const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
     floatx4 acc[2][2];
 #pragma unroll
@@ -62,34 +64,29 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
         for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
 
     for (int k0 = kbeg; k0 < kend; k0 += kBK) {
-        float av[4], bv[4];
+        u16x8 av, bv;
         const int am = m0 + lr, bn = n0 + lr;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < 8; ++u) {
             const int k = k0 + lk + u;
-            av[u] = (am < g.M && k < kend) ? A[am * g.sAi + k * g.sAk] : 0.f;
-            bv[u] = (bn < g.N && k < kend) ? B[k * g.sBk + bn * g.sBj] : 0.f;
+            av[u] = to_bf16((am < g.M && k < kend) ? A[am * g.sAi + k * g.sAk] : 0.f);
+            bv[u] = to_bf16((bn < g.N && k < kend) ? B[k * g.sBk + bn * g.sBj] : 0.f);
         }
         __syncthreads();   // the previous step's operands have been read
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            As[lr * kLdsK + lk + u] = to_bf16(av[u]);
-            Bs[lr * kLdsK + lk + u] = to_bf16(bv[u]);
-        }
+        *reinterpret_cast<u16x8*>(As + lr * kLdsK + lk) = av;
+        *reinterpret_cast<u16x8*>(Bs + lr * kLdsK + lk) = bv;
         __syncthreads();
-        const int fr = lane & 15, fk = (lane >> 4) * 4;
-        bf16x4 a[2], b[2];
+        const int fr = lane & 15, fk = (lane >> 4) * 8;
+        bf16x8 a[2], b[2];
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
-            const unsigned short* pa = As + (wm + i * 16 + fr) * kLdsK + fk;
-            const unsigned short* pb = Bs + (wn + i * 16 + fr) * kLdsK + fk;
-            a[i] = bf16x4{(short)pa[0], (short)pa[1], (short)pa[2], (short)pa[3]};
-            b[i] = bf16x4{(short)pb[0], (short)pb[1], (short)pb[2], (short)pb[3]};
+            a[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(As + (wm + i * 16 + fr) * kLdsK + fk));
+            b[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(Bs + (wn + i * 16 + fr) * kLdsK + fk));
         }
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
-            for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+            for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
     }
 
     const int col = lane & 15, rbase = (lane >> 4) * 4;
