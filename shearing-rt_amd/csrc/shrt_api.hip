@@ -81,7 +81,6 @@ struct shrt_ctx {
     int* jump_tmp = nullptr;  // [nkx]
     HalvingRec* hlog = nullptr;
     double* cflp = nullptr;   // [P2 workgroups][2] CFL partial maxima of the x pass
-    void* fargs = nullptr;    // [2 parities][2 aphi flags] argument blocks of the fused kernel
     Variants var;             // kernel variant of each pass (shrt_autotune)
     double* diag_partial = nullptr;
     double* diag_out = nullptr;
@@ -184,11 +183,6 @@ hipError_t enqueue_step(shrt_ctx* c, int in_par, int store_phi) {
     return c->prec == 4 ? enqueue_step_t<float>(c, in_par, store_phi) : enqueue_step_t<double>(c, in_par, store_phi);
 }
 
-// Whether the fused kernel of a context takes its arguments from device memory:
-// at FP32 2048, where the kernel spills registers with its argument block passed
-// by value (measured on MI355X: elsewhere by value is as fast or faster).
-inline bool fused_args_on_device(const shrt_ctx* c) { return c->prec == 4 && c->g.ny == 2048; }
-
 // Step k of a captured batch of K >= 2 steps.  Fused: step 0 runs P1, P2 and
 // P3+P1 of step 1; steps 1..K-2 run P2 and P3+P1 of the next step; step K-1
 // runs P2 and P3.  aphi is stored for the batch's last step (by the fused
@@ -205,38 +199,15 @@ hipError_t enqueue_batch_step_t(shrt_ctx* c, int in_par, int k, int K, hipStream
     }
     const int sp = k + 1 == K - 1;
     const StepArgs<T> a = step_args<T>(c, in_par, 0, sp);
-    // At FP32 2048 the fused kernel reads its arguments from the device block
-    // written by fill_fused_args: passed by value, its argument block took
-    // registers that the kernel then spilled (2048^2 FP32 fused step 80.7
-    // against 78.2 us).  Elsewhere by value is as fast or faster (1024^2 35.7
-    // against 37.5 us, FP64 2048^2 195 against 212 us).
-    const StepArgs<T>* fa = fused_args_on_device(c) ? static_cast<const StepArgs<T>*>(c->fargs) + 2 * in_par + sp
-                                                    : nullptr;
     hipError_t e = k == 0 ? Launch<T>::p1(a, nullptr, 1, c->var.p1, s) : hipSuccess;
     if (e == hipSuccess) e = Launch<T>::p2(a, nullptr, 1, c->var.p2, s);
-    if (e == hipSuccess) e = last ? Launch<T>::p3(a, nullptr, 1, c->var.p3, s) : Launch<T>::p31(a, fa, 1, s);
-    return e;
-}
-
-// The fused kernel's argument blocks of a context, [in_par][store_phi_next]
-// (step_args of every combination a captured batch launches it with).
-template <typename T>
-hipError_t fill_fused_args(shrt_ctx* c) {
-    StepArgs<T> h[4];
-    for (int ip = 0; ip < 2; ++ip)
-        for (int sp = 0; sp < 2; ++sp) h[2 * ip + sp] = step_args<T>(c, ip, 0, sp);
-    hipError_t e = hipStreamSynchronize(c->stream);   // no running graph reads the blocks
-    if (e == hipSuccess) e = hipMemcpy(c->fargs, h, sizeof(h), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = last ? Launch<T>::p3(a, nullptr, 1, c->var.p3, s) : Launch<T>::p31(a, nullptr, 1, s);
     return e;
 }
 
 shrt_status build_graph(shrt_ctx* c, int par) {
     if (c->gexec[par]) return SHRT_OK;
     const int K = c->p.graph_steps;
-    if (c->var.fuse && !c->ph.narrow && fused_args_on_device(c)) {
-        const hipError_t e = c->prec == 4 ? fill_fused_args<float>(c) : fill_fused_args<double>(c);
-        if (e != hipSuccess) return fail(SHRT_ERR_DEVICE, std::string("fused arguments: ") + hipGetErrorString(e));
-    }
     hipGraph_t graph = nullptr;
     HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     hipError_t err = hipSuccess;
@@ -601,9 +572,9 @@ shrt_status shrt_create(const shrt_params* p, shrt_ctx** out) {
     if (g.ny >= 4096 || (g.ny == 2048 && c->prec == 4)) c->var.p1 = kP1Pair;
     else if (g.ny == 512) c->var.p1 = kP1Tile;
     // Captured batches fuse each step's P3 with the next step's P1 by default
-    // where that measured faster on MI355X: FP32 at 512, 1024 and 2048 (1024^2
-    // 35.1 against 36.9 us per step, 2048^2 78.3 against 80.5-82.2); slower at
-    // 256, at 4096 and in FP64.
+    // where that measured faster on MI355X: FP32 at 512, 1024 and 2048 (512^2
+    // 23.1 against 23.4 us per step, 1024^2 36.1 against 37.1-37.5, 2048^2
+    // 77.0-77.2 against 80.8-81.9, one box); slower at 256, at 4096 and in FP64.
     if (c->prec == 4 && (g.ny == 512 || g.ny == 1024 || g.ny == 2048)) c->var.fuse = 1;
 
     auto bail = [&](shrt_status st) {
@@ -619,7 +590,6 @@ shrt_status shrt_create(const shrt_params* p, shrt_ctx** out) {
         for (int s = 0; s < 3; ++s)
             if ((st = dalloc(c, &c->ring[r][s], M * c->esz)) != SHRT_OK) return bail(st);
     if ((st = dalloc(c, &c->aphi, M * c->esz)) != SHRT_OK) return bail(st);
-    if ((st = dalloc(c, &c->fargs, 4 * sizeof(StepArgs<double>))) != SHRT_OK) return bail(st);
     if ((st = dalloc(c, &c->G, 6 * (size_t)g.I * c->esz)) != SHRT_OK) return bail(st);
     if ((st = dalloc(c, &c->H, 2 * (size_t)g.I * c->esz)) != SHRT_OK) return bail(st);
     if ((st = dalloc(c, &c->tmp_spec, M * c->esz)) != SHRT_OK) return bail(st);
@@ -948,10 +918,6 @@ shrt_status shrt_fused_kernel_time(shrt_ctx* c, int nsteps, double* ms) {
     // launch it (P1; then P2 and P3+P1 per step; P2 and P3 last), each fused
     // dispatch recording events at its own start and end
     const int K = nsteps + 1;
-    if (fused_args_on_device(c)) {
-        const hipError_t e = c->prec == 4 ? fill_fused_args<float>(c) : fill_fused_args<double>(c);
-        if (e != hipSuccess) return fail(SHRT_ERR_DEVICE, std::string("fused arguments: ") + hipGetErrorString(e));
-    }
     std::vector<hipEvent_t> ev((size_t)nsteps * 2);
     for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
     hipError_t err = hipSuccess;
@@ -963,14 +929,12 @@ shrt_status shrt_fused_kernel_time(shrt_ctx* c, int nsteps, double* ms) {
         auto go = [&](auto tag) -> hipError_t {
             using T = decltype(tag);
             const StepArgs<T> a = step_args<T>(c, in_par, 0, sp);
-            const StepArgs<T>* fa = fused_args_on_device(c)
-                                        ? static_cast<const StepArgs<T>*>(c->fargs) + 2 * in_par + sp : nullptr;
             hipError_t e = k == 0 ? Launch<T>::p1(a, nullptr, 1, c->var.p1, c->stream) : hipSuccess;
             if (e == hipSuccess) e = Launch<T>::p2(a, nullptr, 1, c->var.p2, c->stream);
             if (e != hipSuccess) return e;
             if (last) return Launch<T>::p3(a, nullptr, 1, c->var.p3, c->stream);
             g_launch_timing = LaunchTiming{ev[(size_t)k * 2], ev[(size_t)k * 2 + 1]};
-            e = Launch<T>::p31(a, fa, 1, c->stream);
+            e = Launch<T>::p31(a, nullptr, 1, c->stream);
             g_launch_timing = LaunchTiming{};
             return e;
         };
@@ -1479,17 +1443,14 @@ hipError_t group_step(shrt_ensemble* e, const std::vector<int>& grp, const void*
     const Variants& v = e->m[grp[0]]->var;   // the group runs its first member's variants
     const bool last = k == e->K - 1;
     const int sp = v.fuse ? (k + 1 == e->K - 1) : last;   // the batch layout of enqueue_batch_step_t
-    const StepArgs<T>* dfused = static_cast<const StepArgs<T>*>(args) + (2 * (k & 1) + sp) * n;
-    // a group of one takes its arguments by value, except in the fused kernel
-    // where that spills registers (fused_args_on_device)
-    const StepArgs<T>* d = n == 1 ? nullptr : dfused;
-    if (n == 1 && !fused_args_on_device(e->m[grp[0]])) dfused = nullptr;
+    // a group of one takes its arguments by value
+    const StepArgs<T>* d = n == 1 ? nullptr : static_cast<const StepArgs<T>*>(args) + (2 * (k & 1) + sp) * n;
     const StepArgs<T> geom = v.fuse ? step_args<T>(e->m[grp[0]], e->m[grp[0]]->cur ^ (k & 1), 0, sp)
                                     : step_args<T>(e->m[grp[0]], e->m[grp[0]]->cur ^ (k & 1), sp, 0);
     hipError_t err = (!v.fuse || k == 0) ? Launch<T>::p1(geom, d, (int)n, v.p1, s) : hipSuccess;
     if (err == hipSuccess) err = Launch<T>::p2(geom, d, (int)n, v.p2, s);
     if (err == hipSuccess)
-        err = (v.fuse && !last) ? Launch<T>::p31(geom, dfused, (int)n, s) : Launch<T>::p3(geom, d, (int)n, v.p3, s);
+        err = (v.fuse && !last) ? Launch<T>::p31(geom, d, (int)n, s) : Launch<T>::p3(geom, d, (int)n, v.p3, s);
     return err;
 }
 

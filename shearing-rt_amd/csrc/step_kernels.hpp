@@ -859,7 +859,7 @@ k_p3(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
     __shared__ cplx<T> slot[2];   // FUSE: the new ky = 0 values of the row and of its mirror
     const Geo& g = a.g;
     const Phys& ph = a.ph;
-    const int team = threadIdx.x / TT, tid = threadIdx.x - team * TT;
+    const int team = KT == 1 ? 0 : (int)threadIdx.x / TT, tid = threadIdx.x - team * TT;
     // the omega and rho workgroups of a row group read each other's rows: keep them on one XCD
     const int qq = xcd_item(blockIdx.x, gridDim.x);
     const int myside = SPLIT ? (qq & 1) : 0;   // SPLIT: the row of the pair this workgroup updates

@@ -513,17 +513,19 @@ def test_remap_large_grid(gpu, prec, n):
 
 
 @pytest.mark.parametrize("prec", [8, 4])
-@pytest.mark.parametrize("nx,ny,linear", [(32, 32, 0), (64, 128, 0), (128, 128, 0), (32, 32, 1)])
+@pytest.mark.parametrize("nx,ny,linear", [(32, 32, 0), (64, 128, 0), (128, 128, 0), (32, 32, 1), (2048, 2048, 0)])
 def test_fused_batches_with_remaps(gpu, prec, nx, ny, linear):
     """Captured batches run P3 of step n and P1 of step n+1 as one kernel.  With
     sigma*delt*kx[1] = 0.2 a remap fires every fifth step (indices 4, 9, 14, 19):
     at every position of a four-step batch, so the fused kernel forms the remap
     view of the rows it has just updated (and the last step of a batch hands a
     pending remap to the next batch's P1).  Oracle parity, index work
-    bit-exact, and the unfused batches and single steps agree to rounding."""
+    bit-exact, and the unfused batches and single steps agree to rounding;
+    also at the headline size 2048^2, where FP32 contexts run fused by
+    default."""
     print(f"fused batches nx={nx} ny={ny} linear={linear} prec={prec}", flush=True)
     p = dict(sigma=20.0, delt=0.01, cfl_num=1e3, nu=1e-2, kappa=1e-2, linear=linear)
-    o = oracle.Oracle(nx=nx, ny=ny, **dict(BASE, **p))
+    o = oracle.Oracle(nx=nx, ny=ny, threads=16 if nx >= 1024 else 1, **dict(BASE, **p))
     ic = dict(seed=17, kmin=1.0, kmax=8.0, amp=1e-3, expo=-1.0)
     o.set_spectral("rho", oracle.seeded_spectrum(nx, ny, 2 * np.pi, 2 * np.pi, **ic))
     steps = 22
