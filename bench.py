@@ -65,6 +65,16 @@ def fused_bytes(s, c):
     return c * (8 * s["I"] + 12 * s["M"])
 
 
+def step_bytes(s, c, fused, G):
+    """Algorithmic HBM bytes per step of the kernels a captured batch of G
+    steps launches: P1, P2, P3 per step, or fused one P1, G P2, G-1 fused
+    P3+P1 and one P3."""
+    kb = kernel_bytes(s, c)
+    if not fused or G < 2:
+        return sum(kb.values())
+    return kb["p2"] + ((G - 1) * fused_bytes(s, c) + kb["p1"] + kb["p3"]) / G
+
+
 def read_traffic(name):
     """HBM bytes per launch of a step kernel as two rocprofv3 --pmc runs of this
     bench measured them (FETCH_SIZE, WRITE_SIZE, calibrated by a copy of known
@@ -218,9 +228,6 @@ def run_ensemble(shrt, args):
         n = prm["nx"]
         sizes.append(n)
         points += n * n
-        s = shrt.sizes(n, n)
-        nbytes += sum(kernel_bytes(s, 8).values())
-        nbytes_c14 += 8 * (14 * s["M"] + 10 * s["I"])
     tuned = {}
     for n in sorted(set(sizes)):
         first = members[sizes.index(n)]
@@ -230,6 +237,10 @@ def run_ensemble(shrt, args):
             if m == n:
                 g.set_variants(t["p1"], t["p2"], t["p3"])
                 g.set_fusion(t["fuse"])
+    for n in sizes:
+        s = shrt.sizes(n, n)
+        nbytes += step_bytes(s, 8, tuned[n]["fuse"], args.graph_steps)
+        nbytes_c14 += 8 * (14 * s["M"] + 10 * s["I"])
     log(f"ensemble: {len(members)} instances, per-size choices {tuned}")
     ens = shrt.Ensemble(members, steps_per_graph=args.graph_steps)
     ens.step(args.graph_steps)   # warm-up and capture
@@ -507,12 +518,7 @@ def main():
     achieved = kb_all[dname] / (kms[dname] * 1e-3) / 1e9
     traffic, traffic_src = read_traffic(f"{dname}_{n}_{args.precision}")
     ms_step = ms / args.steps
-    if kt31 is not None:
-        # a fused batch of G steps: one P1, G P2, G-1 fused P3+P1, one P3
-        G = max(1, min(args.graph_steps, args.steps))
-        step_bytes = kb["p2"] + ((G - 1) * kb_all["p31"] + kb["p1"] + kb["p3"]) / G
-    else:
-        step_bytes = sum(kb.values())
+    nbytes = step_bytes(s, c, kt31 is not None, min(args.graph_steps, args.steps))
     baseline_formula = c * (14 * s["M"] + 10 * s["I"])
 
     out = {
@@ -539,9 +545,9 @@ def main():
         "kernels_ms": kms,
         "stream_copy_GBs": copy_bw,
         "stream_copy_nt_GBs": scopy_bw,
-        "step_roofline": {"bytes_per_step": step_bytes,
-                          "frac": step_bytes / (ms_step * 1e-3) / 1e9 / PEAK_HBM_GBS,
-                          "frac_of_stream_copy_nt": step_bytes / (ms_step * 1e-3) / 1e9 / scopy_bw,
+        "step_roofline": {"bytes_per_step": nbytes,
+                          "frac": nbytes / (ms_step * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                          "frac_of_stream_copy_nt": nbytes / (ms_step * 1e-3) / 1e9 / scopy_bw,
                           "bytes_per_step_c14M10I": baseline_formula,
                           "frac_c14M10I": baseline_formula / (ms_step * 1e-3) / 1e9 / PEAK_HBM_GBS},
         "final_state": {"istep": st["istep"], "time": st["time"], "delt": st["delt"],

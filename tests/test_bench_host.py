@@ -70,6 +70,18 @@ def test_fused_kernel_bytes_and_counter_key(tmp_path):
     assert out["p31_2048_fp32"]["bytes"] == pytest.approx((2 * 30.0 + 20.0) * 1024)
 
 
+def test_step_bytes_of_a_batch():
+    # a fused batch of G steps launches P1 and P3 once, P2 G times, the fused kernel G - 1 times
+    s = {"M": 1365 * 683, "I": 683 * 2048}
+    kb = bench.kernel_bytes(s, 8)
+    assert bench.step_bytes(s, 8, False, 100) == sum(kb.values()) == 283_461_392
+    assert bench.step_bytes(s, 8, True, 1) == sum(kb.values())
+    G = 100
+    total = kb["p1"] + kb["p3"] + G * kb["p2"] + (G - 1) * bench.fused_bytes(s, 8)
+    assert bench.step_bytes(s, 8, True, G) == pytest.approx(total / G)
+    assert bench.step_bytes(s, 8, True, G) == pytest.approx(268_693_839.2)
+
+
 def test_pmc_summary_needs_the_copy(tmp_path):
     _csv(tmp_path / "x.csv", [("void shrt::k_p2<float, 2048, false, true>(a)", 1.0)])
     with pytest.raises(RuntimeError):
