@@ -158,31 +158,33 @@ __device__ __forceinline__ T wave_max(T v) {
 // workgroup does this; `red` is workgroup-shared scratch of 2*16 doubles).
 // One (max|vx + sigma*y|, max|vy|) pair per P2 workgroup in the working
 // precision (the maxima of working-precision values, so exact), eight loads
-// in flight per thread.
-template <typename T>
+// in flight per thread; NT = threads per workgroup.
+template <typename T, int NT>
 __device__ __forceinline__ void cfl_maxima(const double* __restrict__ cflp, int n, double* red, double* cvx,
                                            double* cvy) {
     const cplx<T>* p = reinterpret_cast<const cplx<T>*>(cflp);
     T m0 = 0, m1 = 0;
     int i = threadIdx.x;
-    for (; i + 7 * (int)blockDim.x < n; i += 8 * blockDim.x) {
+    for (; i + 7 * NT < n; i += 8 * NT) {
         cplx<T> v[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) v[u] = p[i + u * blockDim.x];
+        for (int u = 0; u < 8; ++u) v[u] = p[i + u * NT];
 #pragma unroll
         for (int u = 0; u < 8; ++u) { m0 = fmax(m0, v[u].x); m1 = fmax(m1, v[u].y); }
     }
-    for (; i < n; i += blockDim.x) {
+    for (; i < n; i += NT) {
         const cplx<T> v = p[i];
         m0 = fmax(m0, v.x);
         m1 = fmax(m1, v.y);
     }
     m0 = wave_max(m0);
     m1 = wave_max(m1);
-    const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    constexpr int nw = (NT + 63) >> 6;
+    const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) { red[w] = m0; red[16 + w] = m1; }
     __syncthreads();
     double r0 = 0.0, r1 = 0.0;
+#pragma unroll
     for (int i = 0; i < nw; ++i) { r0 = fmax(r0, red[i]); r1 = fmax(r1, red[16 + i]); }
     *cvx = r0;
     *cvy = r1;

@@ -922,7 +922,7 @@ k_p3(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
     // right after a remap (VIEW), a plain load otherwise.
     auto ld = [&](auto view, const cplx<T>* buf, int r, int ik, const int* jmp) -> cplx<T> {
         if constexpr (decltype(view)::value) return view_load(buf, r, ik, g, jmp);
-        else return buf[(long long)r * g.nky + ik];
+        else return buf[(long long)r * NKY + ik];
     };
     auto load_batch = [&](auto view, auto bc, Batch& B) {
         constexpr int b = decltype(bc)::value, j0 = (b % NBS) * JB;
@@ -970,12 +970,12 @@ k_p3(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
             }
             if constexpr (sizeof(T) == 4 && kWideUpdate && !NARROW) {
                 // FP32 state: the AB/BDF2 combination and the implicit diffusion in FP64, rounded once
-                const uint32_t key = step_key + (uint32_t)(r * g.nky + iky) * 0x9e3779b1u;
+                const uint32_t key = step_key + (uint32_t)(r * NKY + iky) * 0x9e3779b1u;
                 an = ab_bdf_update_wide(ist, plain, dt, diff, k2, B.a1[u], B.a2[u], d0, B.d1[u], B.d2[u], key);
             } else {
                 an = ab_bdf_update<T>(ist, plain, dt, diff, k2, B.a1[u], B.a2[u], d0, B.d1[u], B.d2[u]);
             }
-            const long long at = (long long)r * g.nky + iky;
+            const long long at = (long long)r * NKY + iky;
             if (writer) {
                 A0[at] = an;
                 D0[at] = d0;
@@ -991,7 +991,7 @@ k_p3(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
         Batch bt[2];
 
         // the step size after the CFL halvings, and (workgroup 0) the bookkeeping
-        cfl_maxima<T>(a.cflp, a.ncflp, red, &cvx, &cvy);
+        cfl_maxima<T, NT>(a.cflp, a.ncflp, red, &cvx, &cvy);
         const int h = cfl_delt(c, g, ph, cvx, cvy, &dt, &which);
         if (blockIdx.x == 0) p3_bookkeeping(a, dt, h, which, cvx, cvy);
 
