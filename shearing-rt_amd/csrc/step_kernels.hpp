@@ -637,10 +637,12 @@ k_p2(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
 // P3
 // ---------------------------------------------------------------------------
 // Explicit step + implicit diffusion for one mode (time_integral.cu:621-695,
-// 747-767), in the reference's operation order.
+// 747-767), in the reference's operation order.  dt12 = dt / 12 and dt9 =
+// dt / 9 come from the caller, formed once per workgroup.
 template <typename T>
-__device__ __forceinline__ cplx<T> ab_bdf_update(int ist, bool plain, double dt, double diff, double k2,
-                                                 cplx<T> a1, cplx<T> a2, cplx<T> d0, cplx<T> d1, cplx<T> d2) {
+__device__ __forceinline__ cplx<T> ab_bdf_update(int ist, bool plain, double dt, double dt12, double dt9,
+                                                 double diff, double k2, cplx<T> a1, cplx<T> a2, cplx<T> d0,
+                                                 cplx<T> d1, cplx<T> d2) {
     cplx<T> r;
     if (ist == 0) {                                            // eulerf
         const T c = (T)dt;
@@ -658,12 +660,12 @@ __device__ __forceinline__ cplx<T> ab_bdf_update(int ist, bool plain, double dt,
         }
     } else {
         if (plain) {                                           // ab3
-            const T c = (T)(dt / 12.0);
+            const T c = (T)dt12;
             const cplx<T> w = {(T(23) * d0.x - T(16) * d1.x) + T(5) * d2.x,
                                (T(23) * d0.y - T(16) * d1.y) + T(5) * d2.y};
             r = {a1.x + c * w.x, a1.y + c * w.y};
         } else {                                               // ab3, BDF2 form
-            const T c = (T)(dt / 9.0);
+            const T c = (T)dt9;
             const cplx<T> u = {(T(4) * a1.x - a2.x) / T(3), (T(4) * a1.y - a2.y) / T(3)};
             const cplx<T> w = {(T(16) * d0.x - T(14) * d1.x) + T(4) * d2.x,
                                (T(16) * d0.y - T(14) * d1.y) + T(4) * d2.y};
@@ -717,7 +719,8 @@ __device__ __forceinline__ float round_stochastic(double x, uint32_t key) {
 // stochastic rounding keyed by (step, field, mode): the divisions by 3 and by
 // the diffusion denominator become one reciprocal and products (their FP64
 // rounding is far below the final FP32 rounding).
-__device__ __forceinline__ cplx<float> ab_bdf_update_wide(int ist, bool plain, double dt, double diff, double k2,
+__device__ __forceinline__ cplx<float> ab_bdf_update_wide(int ist, bool plain, double dt, double dt12, double dt9,
+                                                         double diff, double k2,
                                                          cplx<float> a1f, cplx<float> a2f, cplx<float> d0f,
                                                          cplx<float> d1f, cplx<float> d2f, uint32_t key) {
     const cplx<double> a1 = {a1f.x, a1f.y}, a2 = {a2f.x, a2f.y};
@@ -736,10 +739,10 @@ __device__ __forceinline__ cplx<float> ab_bdf_update_wide(int ist, bool plain, d
         }
     } else {
         if (plain) {
-            const double c = dt / 12.0;
+            const double c = dt12;
             r = {a1.x + c * ((23.0 * d0.x - 16.0 * d1.x) + 5.0 * d2.x), a1.y + c * ((23.0 * d0.y - 16.0 * d1.y) + 5.0 * d2.y)};
         } else {
-            const double c = dt / 9.0;
+            const double c = dt9;
             r = {(4.0 * a1.x - a2.x) * third + c * ((16.0 * d0.x - 14.0 * d1.x) + 4.0 * d2.x),
                  (4.0 * a1.y - a2.y) * third + c * ((16.0 * d0.y - 14.0 * d1.y) + 4.0 * d2.y)};
         }
@@ -891,7 +894,7 @@ k_p3(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
     const int* jd2 = c->slot_epoch_d[s2] != c->epoch ? jbase : nullptr;
     const int ist = n == 0 ? 0 : (n == 1 ? 1 : 2);
     const T scale = (T)g.scale;
-    double dt = 0.0, cvx = 0.0, cvy = 0.0;
+    double dt = 0.0, dt12 = 0.0, dt9 = 0.0, cvx = 0.0, cvy = 0.0;
     int which = -1;
 
     cplx<T>* A0 = f ? a.a_rho[s0] : a.a_omg[s0];
@@ -971,9 +974,10 @@ k_p3(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
             if constexpr (sizeof(T) == 4 && kWideUpdate && !NARROW) {
                 // FP32 state: the AB/BDF2 combination and the implicit diffusion in FP64, rounded once
                 const uint32_t key = step_key + (uint32_t)(r * NKY + iky) * 0x9e3779b1u;
-                an = ab_bdf_update_wide(ist, plain, dt, diff, k2, B.a1[u], B.a2[u], d0, B.d1[u], B.d2[u], key);
+                an = ab_bdf_update_wide(ist, plain, dt, dt12, dt9, diff, k2, B.a1[u], B.a2[u], d0, B.d1[u], B.d2[u],
+                                        key);
             } else {
-                an = ab_bdf_update<T>(ist, plain, dt, diff, k2, B.a1[u], B.a2[u], d0, B.d1[u], B.d2[u]);
+                an = ab_bdf_update<T>(ist, plain, dt, dt12, dt9, diff, k2, B.a1[u], B.a2[u], d0, B.d1[u], B.d2[u]);
             }
             const long long at = (long long)r * NKY + iky;
             if (writer) {
@@ -993,6 +997,8 @@ k_p3(StepArgs<T> a1, const StepArgs<T>* __restrict__ args) {
         // the step size after the CFL halvings, and (workgroup 0) the bookkeeping
         cfl_maxima<T, NT>(a.cflp, a.ncflp, red, &cvx, &cvy);
         const int h = cfl_delt(c, g, ph, cvx, cvy, &dt, &which);
+        dt12 = dt / 12.0;   // the AB3 weights' factors, once per workgroup
+        dt9 = dt / 9.0;
         if (blockIdx.x == 0) p3_bookkeeping(a, dt, h, which, cvx, cvy);
 
         if (!ph.linear) {
