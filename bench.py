@@ -214,14 +214,47 @@ def ensemble_members(spec, seed=7):
     return out
 
 
+# Variant choices (P1, P3, fusion) tried per grid size on an ensemble's batched
+# group: P1 0 row, 1 four-row tile, 2 pair; P3 0 pair, 1 tile, 2 split.
+ENSEMBLE_CANDIDATES = [(0, 0, 0), (1, 0, 0), (2, 0, 0), (0, 1, 0), (1, 1, 0), (2, 1, 0), (0, 0, 1), (0, 1, 1)]
+
+
+def tune_ensemble_group(shrt, group, graph_steps, steps=100):
+    """The variant choice of one grid size for a batched ensemble: each
+    candidate of ENSEMBLE_CANDIDATES times `steps` ensemble steps of scratch
+    instances built from the group's (params, initial condition) list; returns
+    (p1, p3, fuse) of the fastest and the times.  Single-instance tuning
+    (shrt_autotune) optimises one instance's launch latency; a batched group of
+    small grids is bound by its workgroups' lane occupancy instead (the tile
+    variants give whole wavefronts below 512^2)."""
+    times = {}
+    for p1, p3, fuse in ENSEMBLE_CANDIDATES:
+        gs = []
+        for prm, ic in group:
+            g = shrt.Shrt(precision=shrt.FP32, graph_steps=0, **prm)
+            g.initialize_seeded(**ic)
+            g.set_variants(p1, 0, p3)
+            g.set_fusion(fuse)
+            gs.append(g)
+        e = shrt.Ensemble(gs, steps_per_graph=graph_steps)
+        e.step(graph_steps)
+        times[(p1, p3, fuse)] = e.time_steps(steps) / steps
+        e.close()
+        for g in gs:
+            g.close()
+    return min(times, key=times.get), times
+
+
 def run_ensemble(shrt, args):
     """Aggregate throughput of independent instances stepped by one captured
     graph per batch (shrt_ensemble_*), members from ensemble_members.  The
-    autotuner picks the variants of each grid size once (on that size's first
-    instance) for all its members.  The roofline fraction sums each member's
-    algorithmic bytes per step over the ensemble's time per step."""
+    variants of each grid size are tuned on that size's batched group
+    (tune_ensemble_group) and set on all its members.  The roofline fraction
+    sums each member's algorithmic bytes per step over the ensemble's time per
+    step."""
     members, points, sizes, nbytes, nbytes_c14 = [], 0, [], 0, 0
-    for prm, ic in ensemble_members(args.ensemble):
+    specs = ensemble_members(args.ensemble)
+    for prm, ic in specs:
         g = shrt.Shrt(precision=shrt.FP32, graph_steps=0, **prm)
         g.initialize_seeded(**ic)
         members.append(g)
@@ -230,13 +263,14 @@ def run_ensemble(shrt, args):
         points += n * n
     tuned = {}
     for n in sorted(set(sizes)):
-        first = members[sizes.index(n)]
-        t = first.autotune(reps=10)
-        tuned[n] = dict(variants=[t["p1"], t["p2"], t["p3"]], fuse=t["fuse"])
+        (p1, p3, fuse), times = tune_ensemble_group(shrt, [sp for sp, m in zip(specs, sizes) if m == n],
+                                                    args.graph_steps)
+        tuned[n] = dict(variants=[p1, 0, p3], fuse=fuse,
+                        ms_per_step={f"{a},{b},{c}": round(t, 5) for (a, b, c), t in times.items()})
         for g, m in zip(members, sizes):
             if m == n:
-                g.set_variants(t["p1"], t["p2"], t["p3"])
-                g.set_fusion(t["fuse"])
+                g.set_variants(p1, 0, p3)
+                g.set_fusion(fuse)
     for n in sizes:
         s = shrt.sizes(n, n)
         nbytes += step_bytes(s, 8, tuned[n]["fuse"], args.graph_steps)

@@ -37,11 +37,17 @@ constexpr int waves_per_eu() {
 // P1/P3 teams), the P1/P3 team shape below.
 template <typename T, int N> constexpr int p2_points() { return N >= 2048 ? 16 : TeamShape<T, N>::P; }
 // Columns (teams) per P2 workgroup: 1, 2 or kTile.  Two at FP64 2048 (P2
-// 57.9 us per launch against 66.5 with one column); one elsewhere (two columns
-// measured 2048^2 FP32 28.7 against 29.0 us, FP32 1024 13.0 against 12.1, FP64
-// 1024 23.1 against 22.0, FP64 4096 341 against 311; four columns at 2048^2
-// FP32 31.5 us).
-template <typename T, int N> constexpr int p2_teams() { return (sizeof(T) == 8 && N == 2048) ? 2 : 1; }
+// 57.9 us per launch against 66.5 with one column); below 512, where a team
+// is less than a wavefront, as many as fill one (at most kTile); one elsewhere
+// (two columns measured 2048^2 FP32 28.7 against 29.0 us, FP32 1024 13.0
+// against 12.1, FP64 1024 23.1 against 22.0, FP64 4096 341 against 311; four
+// columns at 2048^2 FP32 31.5 us).
+template <typename T, int N> constexpr int p2_teams() {
+    constexpr int tt = Team<N, p2_points<T, N>()>::TT;
+    if (sizeof(T) == 8 && N == 2048) return 2;
+    if (tt < 64) return 64 / tt < kTile ? 64 / tt : kTile;
+    return 1;
+}
 
 // Kernel variants chosen per context at run time (shrt_autotune): one code per
 // pass.  Defaults: the fastest at 2048^2 FP32 (P1 31.3 us row against 32.4 us

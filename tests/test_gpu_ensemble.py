@@ -112,3 +112,40 @@ def test_config4_mixed_ensemble_256(gpu):
         ens.close()
         for g in members:
             g.close()
+
+
+@pytest.mark.parametrize("p1,p3", [(1, 1), (2, 1)])
+def test_config4_ensemble_with_tuned_variants(gpu, p1, p3):
+    """The same 256-instance draw with the variant choices the batched tuning
+    of bench.py --ensemble makes (four-row tiles, or row pairs for P1, and
+    four-row tiles for P3, unfused; P2 with several columns per workgroup below
+    512^2), 200 steps: every instance equals its own unbatched run with the
+    same variants bit for bit."""
+    steps = 200
+    draw = bench.ensemble_members("mixed:256")
+    print(f"config-4 ensemble, variants p1={p1} p3={p3}: {len(draw)} instances", flush=True)
+    members = []
+    for prm, ic in draw:
+        g = shrt.Shrt(precision=shrt.FP32, graph_steps=0, **prm)
+        g.initialize_seeded(**ic)
+        g.set_variants(p1, 0, p3)
+        g.set_fusion(False)
+        members.append(g)
+    ens = shrt.Ensemble(members, steps_per_graph=100)
+    try:
+        ens.step(steps)
+        for i, ((prm, ic), a) in enumerate(zip(draw, members)):
+            b = shrt.Shrt(precision=shrt.FP32, graph_steps=100, **prm)
+            b.initialize_seeded(**ic)
+            b.set_variants(p1, 0, p3)
+            b.set_fusion(False)
+            b.time_advance(steps)
+            for f in (shrt.OMEGA, shrt.RHO, shrt.PHI):
+                assert np.array_equal(a.spectral(f).view(np.uint32), b.spectral(f).view(np.uint32)), (i, prm, f)
+            assert a.state() == b.state(), (i, prm)
+            b.close()
+        print("every instance equal to its unbatched run", flush=True)
+    finally:
+        ens.close()
+        for g in members:
+            g.close()
