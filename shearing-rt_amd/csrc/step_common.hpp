@@ -40,8 +40,10 @@ template <typename T, int N> constexpr int p2_points() { return N >= 2048 ? 16 :
 // 57.9 us per launch against 66.5 with one column); below 512, where a team
 // is less than a wavefront, as many as fill one (at most kTile); one elsewhere
 // (two columns measured 2048^2 FP32 28.7 against 29.0 us, FP32 1024 13.0
-// against 12.1, FP64 1024 23.1 against 22.0, FP64 4096 341 against 311; four
-// columns at 2048^2 FP32 31.5 us).
+// against 12.1, FP64 1024 23.1 against 22.0, FP64 4096 341 against 311, a
+// single 512^2 instance's step FP32 23.4 against 22.6 and FP64 33.9 against
+// 32.0 though a batch of 89 such instances 5% faster; four columns at 2048^2
+// FP32 31.5 us).
 template <typename T, int N> constexpr int p2_teams() {
     constexpr int tt = Team<N, p2_points<T, N>()>::TT;
     if (sizeof(T) == 8 && N == 2048) return 2;
