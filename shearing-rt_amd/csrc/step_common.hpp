@@ -34,12 +34,10 @@ constexpr int waves_per_eu() {
 
 // Points per thread of the P2 transforms: 16 from 2048 up in both precisions
 // (FP64 2048^2: 77.5 us per launch against 87.2 us with the 8 points of its
-// P1/P3 teams); 4 for FP64 at 1024 (256-thread columns, twice the waves: step
-// 51.3-51.6 against 52.1 us; FP32 35.8 against 35.1, so FP32 keeps 8); the
-// P1/P3 team shape below otherwise.
-template <typename T, int N> constexpr int p2_points() {
-    return N >= 2048 ? 16 : ((N == 1024 && sizeof(T) == 8) ? 4 : TeamShape<T, N>::P);
-}
+// P1/P3 teams); the P1/P3 team shape below 2048 (4 points at FP64 1024, where
+// the 256-thread columns alone took the step from 52.1 to 51.3-51.6 us; in FP32
+// 4 points were slower, 35.8 against 35.1).
+template <typename T, int N> constexpr int p2_points() { return N >= 2048 ? 16 : TeamShape<T, N>::P; }
 // Columns (teams) per P2 workgroup: 1, 2 or kTile.  Two at FP64 2048 (P2
 // 57.9 us per launch against 66.5 with one column); below 512, where a team
 // is less than a wavefront, as many as fill one (at most kTile); one elsewhere
