@@ -311,7 +311,8 @@ struct Team {
 // team at 2048, 4 at 4096) and at 4096 in FP64; 4 at 1024 in FP64; 8 below.
 // Measured per launch on MI355X: 1024^2 FP32 step 37.7 us with 8 points
 // against 45.2 us with 16, FP64 68.9 -> 55.9 us, and FP64 with 4 points 50.1
-// against 51.2 us (4 waves per team: P3 14.5 against 15.1); 2048^2 FP32 with 8
+// against 51.2 us (4 waves per team: P3 14.5 against 15.1), while FP32 with 4
+// points took 40.5 against 35.3 (the fused P3+P1 28.7 against 23.4); 2048^2 FP32 with 8
 // points P1 29.9 against 25.5 us, P3 33.4 against 30.0 (16 kept); 2048^2 FP64
 // with 8 points P1 40.6 against 44.5 us, P3 45.1 against 47.1 (8 taken; P2
 // keeps 16, p2_points).
