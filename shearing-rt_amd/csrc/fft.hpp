@@ -315,8 +315,9 @@ struct Team {
 // points took 40.5 against 35.3 (the fused P3+P1 28.7 against 23.4); 2048^2 FP32 with 8
 // points P1 29.9 against 25.5 us, P3 33.4 against 30.0 (16 kept); 2048^2 FP64
 // with 8 points P1 40.6 against 44.5 us, P3 45.1 against 47.1 (8 taken; P2
-// keeps 16, p2_points); 4096^2 FP32 with 8 points 385-398 against 369 us per
-// step (16 kept).
+// keeps 16, p2_points), with 4 points 164.4 against 156.9 us per step (P3 58.8
+// against 51.2); 4096^2 FP32 with 8 points 385-398 against 369 us per step (16
+// kept).
 template <typename T, int N>
 struct TeamShape {
     static constexpr int P = (sizeof(T) == 4 ? N >= 2048 : N >= 4096) ? 16
